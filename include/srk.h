@@ -1,0 +1,195 @@
+/*
+ * srk.h -- C ABI of the MI355X-native SRK shrinkage operator library
+ * (libsrk_amd.so, gfx950).
+ *
+ *   h = s * ( conv(Gx, g) * df/dx + conv(Gy, g) * df/dy ),   s = dx^2,
+ *
+ * evaluated on the valid region W = N - 2*erosion of column-major N x N
+ * images (element (row r, col c) at c*N + r; batches outermost).
+ *
+ * The four drop-in entry points keep the reference's argument lists, with the
+ * reference's C++ template parameters (Arch, M, EPT, FPB, FPB_COL) and
+ * workspace structs folded into one opaque workspace created for a tile size M.
+ * Spectra are in the reference's layout: Ghat[k_x * (M/2+1) + k_y], complex
+ * float (re, im), unscaled, as produced by srkPrecomputeKernelFreq.
+ *
+ * Semantics that differ from the reference, on purpose:
+ *   - Calls are asynchronous on `stream` and never allocate or synchronise, so
+ *     they can be captured into a hipGraph; call hipStreamSynchronize (or
+ *     srkSynchronize) to get the reference's blocking behaviour.
+ *   - Every function returns an int status (SRK_OK = 0, negative on error)
+ *     instead of void.
+ *   - lambda_g (dLdE / dLam_g) is the true adjoint of the forward convolution
+ *     (a correlation, product with conj(Ghat)), i.e. the reference CPU brute
+ *     force adjoint/main_adjoint.cu:206-246.  The reference GPU adjointWrtE
+ *     (adjoint/srk_adjoint.cuh:301-382) multiplies by the un-conjugated
+ *     spectrum, which equals this result for even kernels and its negative for
+ *     the odd SRSRK kernels.
+ *   - The workspace may be shared by all entry points and batch sizes up to
+ *     B_max (the reference asserts B == B_max, training/srk_training_forward.cuh:186).
+ */
+#ifndef SRK_AMD_H
+#define SRK_AMD_H
+
+#include <stddef.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    SRK_OK = 0,
+    SRK_ERR_INVALID_ARG = -1,      /* negative / inconsistent scalar arguments */
+    SRK_ERR_BAD_DIMS = -2,         /* W != N - 2*erosion, W <= 0, erosion < 1   */
+    SRK_ERR_NULL_PTR = -3,         /* a required pointer is NULL                */
+    SRK_ERR_MISALIGNED = -4,       /* pointer not aligned to its element type   */
+    SRK_ERR_UNSUPPORTED_SIZE = -5, /* tile/FFT size M not in {256,512,1024,2048}*/
+    SRK_ERR_WORKSPACE = -6,        /* workspace made for other N / W / erosion  */
+    SRK_ERR_BATCH = -7,            /* B < 1 or B > workspace B_max              */
+    SRK_ERR_HIP = -8,              /* HIP runtime error (see srkLastHipError)   */
+    SRK_ERR_KERNEL = -9            /* kernel parameters (KW, sigma, dx) invalid */
+};
+
+/* precision of the tile transforms */
+enum {
+    SRK_PREC_FP32 = 0
+};
+
+typedef struct srk_workspace_s* srk_workspace_t;
+
+const char* srkGetErrorString(int status);
+int srkLastHipError(void);
+int srkGetVersion(void);
+
+/* ---- kernel spectra ---------------------------------------------------- */
+
+/*
+ * Replaces srkPrecomputeKernelFreq (srk_forward.cuh:154-203): zero-pad the
+ * kw x kh col-major kernel h_G (host, float) into M x M with its centre at
+ * offset 0 (the reference's pad + fftshift) and write its 2-D DFT (forward,
+ * unnormalised) to d_G_freq[M*(M/2+1)] (device, reference layout).  The DFT is
+ * evaluated in double precision on the host.  Synchronous.
+ */
+int srkPrecomputeKernelFreq(const float* h_G, int kw, int kh, int M, void* d_G_freq,
+                            hipStream_t stream);
+
+/*
+ * SRSRK Green's tensor spectra built entirely in double precision and cached
+ * by (kw, sigma, dx, cw, M): Gx = (g/2pi)(x/r) K1(g r) w(r), Gy likewise, with
+ * g = 1/sigma, Blackman radius Rw = cw*sigma (training/main_cd_training.cu:40-77,
+ * tan_tanadj/main_tan_tanadj.cu:50-116), written in the reference spectrum
+ * layout.  h_Gx / h_Gy (optional, kw*kw floats, col-major) receive the spatial
+ * kernels.  Synchronous.
+ */
+int srkKernelSpectraSRSRK(int kw, double sigma, double dx, double cw, int M, void* d_Gx_freq,
+                          void* d_Gy_freq, float* h_Gx, float* h_Gy, hipStream_t stream);
+
+/* ---- workspace ---------------------------------------------------------- */
+
+/*
+ * Replaces SrkConvWorkspace / SrkAdjWorkspace / SrkTanWorkspace /
+ * SrkTanAdjWorkspace ::allocate (srk_forward.cuh:287-345,
+ * adjoint/srk_adjoint.cuh:232-294, tan_tanadj/srk_tangent.cuh:61-85,
+ * tan_tanadj/srk_tangent_adjoint.cuh:50-79) and SrkTrainWorkspace::allocate
+ * (training/srk_training_forward.cuh:117-158).  One workspace serves every
+ * entry point for images of size N with the given erosion, batches up to
+ * B_max, FFT/tile size M (the size of the spectra passed to the calls; M = N
+ * for srkTrainingForward).  scratch_mb bounds the exchange buffers (0 = 256).
+ */
+int srkWorkspaceCreate(int N, int W, int erosion, int M, int B_max, size_t scratch_mb,
+                       srk_workspace_t* ws);
+int srkWorkspaceDestroy(srk_workspace_t ws);   /* replaces SrkTrainWorkspace::release */
+size_t srkWorkspaceBytes(srk_workspace_t ws);  /* device bytes held */
+int srkWorkspaceSetChunk(srk_workspace_t ws, int tiles_per_chunk); /* 0 = automatic */
+int srkWorkspaceInfo(srk_workspace_t ws, int* M, int* fwd_tiles_per_image,
+                     int* adj_tiles_per_image, int* chunk_fwd, int* chunk_adj);
+
+int srkSynchronize(hipStream_t stream);
+
+/* ---- drop-in entry points ---------------------------------------------- */
+
+/*
+ * Replaces srkTrainingForward (training/srk_training_forward.cuh:175-241):
+ * R_b = dx^2 (conv(E_b,Gx) dI_b/dx + conv(E_b,Gy) dI_b/dy) on the valid region.
+ * E, I: [B x N x N]; R: [B x W x W]; spectra at size N.  E may equal I.
+ * This is also the op_test() contract of training/main_cd_training.cu:101-121.
+ */
+int srkTrainingForward(const float* E, const float* I, const void* Gx_freq, const void* Gy_freq,
+                       int N, int W, int erosion, float dx, int B, hipStream_t stream, float* R,
+                       srk_workspace_t ws);
+
+/*
+ * Replaces srkForwardOaS (srk_forward.cuh:468-501): single-image forward with
+ * E = g (convolved) and I = f (differentiated).  ux, uy (optional, [W x W])
+ * receive s*conv(E,Gx), s*conv(E,Gy) on the valid region when non-NULL.
+ */
+int srkForwardOaS(const float* E, const float* I, const void* Gx_freq, const void* Gy_freq,
+                  int N, int W, int erosion, float dx, hipStream_t stream, float* R,
+                  srk_workspace_t ws, float* ux, float* uy);
+
+/*
+ * Replaces adjointSRK (adjoint/srk_adjoint.cuh:402-425): for cotangent v
+ * [W x W], dLdI = lambda_f [N x N] from the given ux, uy [W x W]
+ * (adjoint/srk_adjoint.cuh:25-75) and dLdE = lambda_g [N x N], the true
+ * adjoint with respect to E (see the header comment).
+ */
+int adjointSRK(const float* v, const float* ux, const float* uy, const float* I,
+               const void* Gx_freq, const void* Gy_freq, int N, int W, int erosion, float dx,
+               hipStream_t stream, float* dLdI, float* dLdE, srk_workspace_t ws);
+
+/*
+ * Replaces shrinkageTangentOaS (tan_tanadj/srk_tangent.cuh:106-151):
+ * dh = s( conv(Gx,dg) f_x + conv(Gy,dg) f_y + conv(Gx,g) df_x + conv(Gy,g) df_y ).
+ */
+int shrinkageTangentOaS(const float* f, const float* g, const float* df, const float* dg,
+                        const void* Gx_freq, const void* Gy_freq, int N, int W, int erosion,
+                        float dx, hipStream_t stream, float* dh, srk_workspace_t ws);
+
+/*
+ * Replaces shrinkageTangentAdjointOaS (tan_tanadj/srk_tangent_adjoint.cuh:100-159):
+ * dLam_f = A_f(dlam_h; u(g)) + A_f(lam_h; u(dg)),
+ * dLam_g = A_g(dlam_h; f) + A_g(lam_h; df)   (A_g the true adjoint).
+ */
+int shrinkageTangentAdjointOaS(const float* f, const float* g, const float* df,
+                               const float* dg, const float* lam_h, const float* dlam_h,
+                               const void* Gx_freq, const void* Gy_freq, int N, int W,
+                               int erosion, float dx, hipStream_t stream, float* dLam_f,
+                               float* dLam_g, srk_workspace_t ws);
+
+/* ---- batched entry points (no reference counterpart) ------------------- */
+
+/*
+ * Batched forward with spectra at the workspace tile size M (any N; N > M
+ * runs overlap-and-save tiles).  g = E (convolved), f = I (differentiated).
+ */
+int srkForwardBatched(const float* E, const float* I, const void* Gx_freq, const void* Gy_freq,
+                      int N, int W, int erosion, float dx, int B, hipStream_t stream, float* R,
+                      srk_workspace_t ws);
+
+/*
+ * Batched adjoint with recomputed activations: (dLdI, dLdE) = (lambda_f,
+ * lambda_g) for cotangents v [B x W x W]; u = s*conv(E, G) is recomputed
+ * from E instead of being stored by the forward.  The adjoint counterpart of
+ * srkTrainingForward; E may equal I.
+ */
+int srkAdjointBatched(const float* v, const float* E, const float* I, const void* Gx_freq,
+                      const void* Gy_freq, int N, int W, int erosion, float dx, int B,
+                      hipStream_t stream, float* dLdI, float* dLdE, srk_workspace_t ws);
+
+/* Batched tangent / tangent-adjoint ([B x ...] arrays, as above). */
+int srkTangentBatched(const float* f, const float* g, const float* df, const float* dg,
+                      const void* Gx_freq, const void* Gy_freq, int N, int W, int erosion,
+                      float dx, int B, hipStream_t stream, float* dh, srk_workspace_t ws);
+int srkTangentAdjointBatched(const float* f, const float* g, const float* df, const float* dg,
+                             const float* lam_h, const float* dlam_h, const void* Gx_freq,
+                             const void* Gy_freq, int N, int W, int erosion, float dx, int B,
+                             hipStream_t stream, float* dLam_f, float* dLam_g,
+                             srk_workspace_t ws);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SRK_AMD_H */

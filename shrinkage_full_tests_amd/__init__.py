@@ -1,0 +1,239 @@
+"""MI355X-native SRK shrinkage operator (gfx950 HIP kernels behind a C ABI).
+
+Host-side mirror of the reference operator interface.  The entry points keep
+the reference names and argument meaning (see include/srk.h); arrays are
+torch tensors on the GPU in the reference's column-major layout, i.e. a
+contiguous tensor of shape [B, N, N] indexed [b, column, row].
+
+Nothing here computes on the CPU: every operator runs the HIP library
+libsrk_amd.so and raises if it is not built.
+"""
+from __future__ import annotations
+
+import ctypes
+
+from ._lib import SrkError, call, lib  # noqa: F401
+
+__all__ = [
+    "SrkError", "lib", "Workspace", "kernel_spectra_srsrk", "precompute_kernel_freq",
+    "srkTrainingForward", "srkForwardOaS", "adjointSRK", "shrinkageTangentOaS",
+    "shrinkageTangentAdjointOaS", "forward_batched", "adjoint_batched", "tangent_batched",
+    "tangent_adjoint_batched", "ShrinkageFunction", "erosion_for", "SUPPORTED_M",
+]
+
+SUPPORTED_M = (256, 512, 1024, 2048)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _ptr(t) -> int:
+    return 0 if t is None else int(t.data_ptr())
+
+
+def _stream(stream) -> int:
+    torch = _torch()
+    if stream is None:
+        return int(torch.cuda.current_stream().cuda_stream)
+    if isinstance(stream, int):
+        return stream
+    return int(stream.cuda_stream)
+
+
+def erosion_for(kw: int) -> int:
+    """Reference default erosion (KW-1)/2 (README.md 'Derived parameters')."""
+    return (kw - 1) // 2
+
+
+class Workspace:
+    """Opaque device workspace for images of size N, erosion e, batches <= B_max,
+    FFT/tile size M (the size of the spectra handed to the calls)."""
+
+    def __init__(self, N: int, W: int, erosion: int, M: int, B_max: int = 1, scratch_mb: int = 0):
+        h = ctypes.c_void_p()
+        call("srkWorkspaceCreate", N, W, erosion, M, B_max, scratch_mb, ctypes.byref(h))
+        self.handle = h.value
+        self.N, self.W, self.erosion, self.M, self.B_max = N, W, erosion, M, B_max
+
+    def close(self):
+        if self.handle:
+            call("srkWorkspaceDestroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def nbytes(self) -> int:
+        return int(lib().srkWorkspaceBytes(self.handle))
+
+    def set_chunk(self, tiles: int) -> None:
+        call("srkWorkspaceSetChunk", self.handle, int(tiles))
+
+    def info(self) -> dict:
+        v = [ctypes.c_int() for _ in range(5)]
+        call("srkWorkspaceInfo", self.handle, *[ctypes.byref(x) for x in v])
+        keys = ("M", "fwd_tiles_per_image", "adj_tiles_per_image", "chunk_fwd", "chunk_adj")
+        return {k: x.value for k, x in zip(keys, v)}
+
+
+def kernel_spectra_srsrk(kw: int, sigma: float, M: int, dx: float = 1.0, cw: float = 3.0,
+                         device="cuda", with_spatial: bool = False):
+    """FP64-built SRSRK spectra (reference layout, complex64 [M, M/2+1]) at tile
+    size M; cached inside the library by (kw, sigma, dx, cw, M)."""
+    torch = _torch()
+    gx = torch.empty((M, M // 2 + 1), dtype=torch.complex64, device=device)
+    gy = torch.empty_like(gx)
+    hx = hy = None
+    if with_spatial:
+        hx = torch.empty((kw, kw), dtype=torch.float32)
+        hy = torch.empty((kw, kw), dtype=torch.float32)
+    call("srkKernelSpectraSRSRK", kw, float(sigma), float(dx), float(cw), M, _ptr(gx), _ptr(gy),
+         _ptr(hx), _ptr(hy), _stream(None))
+    if with_spatial:
+        return gx, gy, hx, hy
+    return gx, gy
+
+
+def precompute_kernel_freq(G, M: int, device="cuda"):
+    """srkPrecomputeKernelFreq of a host [KW, KW] float32 kernel (indexed [x, y])."""
+    torch = _torch()
+    G = G.detach().to("cpu", torch.float32).contiguous()
+    out = torch.empty((M, M // 2 + 1), dtype=torch.complex64, device=device)
+    call("srkPrecomputeKernelFreq", _ptr(G), G.shape[0], G.shape[1], M, _ptr(out), _stream(None))
+    return out
+
+
+# ---- drop-in entry points (reference names) --------------------------------
+
+def srkTrainingForward(E, I, Gx_freq, Gy_freq, N, W, erosion, dx, B, ws, R, stream=None):
+    call("srkTrainingForward", _ptr(E), _ptr(I), _ptr(Gx_freq), _ptr(Gy_freq), N, W, erosion,
+         float(dx), B, _stream(stream), _ptr(R), ws.handle)
+    return R
+
+
+def srkForwardOaS(E, I, Gx_freq, Gy_freq, N, W, erosion, dx, ws, R, ux=None, uy=None,
+                  stream=None):
+    call("srkForwardOaS", _ptr(E), _ptr(I), _ptr(Gx_freq), _ptr(Gy_freq), N, W, erosion,
+         float(dx), _stream(stream), _ptr(R), ws.handle, _ptr(ux), _ptr(uy))
+    return R
+
+
+def adjointSRK(v, ux, uy, I, Gx_freq, Gy_freq, N, W, erosion, dx, ws, dLdI, dLdE, stream=None):
+    call("adjointSRK", _ptr(v), _ptr(ux), _ptr(uy), _ptr(I), _ptr(Gx_freq), _ptr(Gy_freq), N, W,
+         erosion, float(dx), _stream(stream), _ptr(dLdI), _ptr(dLdE), ws.handle)
+    return dLdI, dLdE
+
+
+def shrinkageTangentOaS(f, g, df, dg, Gx_freq, Gy_freq, N, W, erosion, dx, ws, dh, stream=None):
+    call("shrinkageTangentOaS", _ptr(f), _ptr(g), _ptr(df), _ptr(dg), _ptr(Gx_freq),
+         _ptr(Gy_freq), N, W, erosion, float(dx), _stream(stream), _ptr(dh), ws.handle)
+    return dh
+
+
+def shrinkageTangentAdjointOaS(f, g, df, dg, lam_h, dlam_h, Gx_freq, Gy_freq, N, W, erosion, dx,
+                               ws, dLam_f, dLam_g, stream=None):
+    call("shrinkageTangentAdjointOaS", _ptr(f), _ptr(g), _ptr(df), _ptr(dg), _ptr(lam_h),
+         _ptr(dlam_h), _ptr(Gx_freq), _ptr(Gy_freq), N, W, erosion, float(dx), _stream(stream),
+         _ptr(dLam_f), _ptr(dLam_g), ws.handle)
+    return dLam_f, dLam_g
+
+
+# ---- batched entry points ---------------------------------------------------
+
+def forward_batched(E, I, Gx_freq, Gy_freq, erosion, dx, ws, R=None, stream=None):
+    torch = _torch()
+    B, N = E.shape[0], E.shape[-1]
+    W = N - 2 * erosion
+    if R is None:
+        R = torch.empty((B, W, W), dtype=torch.float32, device=E.device)
+    call("srkForwardBatched", _ptr(E), _ptr(I), _ptr(Gx_freq), _ptr(Gy_freq), N, W, erosion,
+         float(dx), B, _stream(stream), _ptr(R), ws.handle)
+    return R
+
+
+def adjoint_batched(v, E, I, Gx_freq, Gy_freq, erosion, dx, ws, dLdI=None, dLdE=None,
+                    stream=None):
+    torch = _torch()
+    B, N = E.shape[0], E.shape[-1]
+    W = N - 2 * erosion
+    if dLdI is None:
+        dLdI = torch.empty((B, N, N), dtype=torch.float32, device=E.device)
+    if dLdE is None:
+        dLdE = torch.empty((B, N, N), dtype=torch.float32, device=E.device)
+    call("srkAdjointBatched", _ptr(v), _ptr(E), _ptr(I), _ptr(Gx_freq), _ptr(Gy_freq), N, W,
+         erosion, float(dx), B, _stream(stream), _ptr(dLdI), _ptr(dLdE), ws.handle)
+    return dLdI, dLdE
+
+
+def tangent_batched(f, g, df, dg, Gx_freq, Gy_freq, erosion, dx, ws, dh=None, stream=None):
+    torch = _torch()
+    B, N = f.shape[0], f.shape[-1]
+    W = N - 2 * erosion
+    if dh is None:
+        dh = torch.empty((B, W, W), dtype=torch.float32, device=f.device)
+    call("srkTangentBatched", _ptr(f), _ptr(g), _ptr(df), _ptr(dg), _ptr(Gx_freq), _ptr(Gy_freq),
+         N, W, erosion, float(dx), B, _stream(stream), _ptr(dh), ws.handle)
+    return dh
+
+
+def tangent_adjoint_batched(f, g, df, dg, lam_h, dlam_h, Gx_freq, Gy_freq, erosion, dx, ws,
+                            dLam_f=None, dLam_g=None, stream=None):
+    torch = _torch()
+    B, N = f.shape[0], f.shape[-1]
+    W = N - 2 * erosion
+    if dLam_f is None:
+        dLam_f = torch.empty((B, N, N), dtype=torch.float32, device=f.device)
+    if dLam_g is None:
+        dLam_g = torch.empty((B, N, N), dtype=torch.float32, device=f.device)
+    call("srkTangentAdjointBatched", _ptr(f), _ptr(g), _ptr(df), _ptr(dg), _ptr(lam_h),
+         _ptr(dlam_h), _ptr(Gx_freq), _ptr(Gy_freq), N, W, erosion, float(dx), B,
+         _stream(stream), _ptr(dLam_f), _ptr(dLam_g), ws.handle)
+    return dLam_f, dLam_g
+
+
+def _make_function():
+    torch = _torch()
+
+    class ShrinkageFunction(torch.autograd.Function):
+        """Autograd pairing of the forward with its hand-written adjoint.
+
+        h = ShrinkageFunction.apply(f, g, Gx_freq, Gy_freq, erosion, dx, ws):
+        the backward recomputes u from g (nothing but f and g is saved), so
+        peak memory stays at the inputs and outputs."""
+
+        @staticmethod
+        def forward(ctx, f, g, Gx_freq, Gy_freq, erosion, dx, ws):
+            f = f.contiguous()
+            g = g.contiguous()
+            ctx.save_for_backward(f, g)
+            ctx.cfg = (Gx_freq, Gy_freq, erosion, dx, ws)
+            return forward_batched(g, f, Gx_freq, Gy_freq, erosion, dx, ws)
+
+        @staticmethod
+        def backward(ctx, v):
+            f, g = ctx.saved_tensors
+            Gx_freq, Gy_freq, erosion, dx, ws = ctx.cfg
+            lam_f, lam_g = adjoint_batched(v.contiguous(), g, f, Gx_freq, Gy_freq, erosion, dx, ws)
+            return lam_f, lam_g, None, None, None, None, None
+
+    return ShrinkageFunction
+
+
+def __getattr__(name):
+    if name == "ShrinkageFunction":
+        fn = _make_function()
+        globals()["ShrinkageFunction"] = fn
+        return fn
+    raise AttributeError(name)

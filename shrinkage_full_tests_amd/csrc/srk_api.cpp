@@ -1,0 +1,607 @@
+// srk_api.cpp -- C ABI of the SRK shrinkage library: workspaces, tile tables,
+// FP64 kernel spectra, and the drop-in entry points declared in include/srk.h.
+#include "srk.h"
+
+#include "srk_engine.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+namespace {
+
+thread_local int g_last_hip = 0;
+
+int hip_status(hipError_t e)
+{
+    if (e != hipSuccess) {
+        g_last_hip = (int)e;
+        return SRK_ERR_HIP;
+    }
+    return SRK_OK;
+}
+int hip_status(int e) { return hip_status((hipError_t)e); }
+
+bool supported_M(int M) { return M == 256 || M == 512 || M == 1024 || M == 2048; }
+
+template <typename T>
+bool aligned(const T* p, size_t a)
+{
+    return (reinterpret_cast<uintptr_t>(p) % a) == 0;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 modified Bessel K1.  x <= 2: ascending series A&S 9.6.11 (n = 1);
+// x > 2: K1(x) = int_0^inf exp(-x cosh t) cosh t dt (A&S 9.6.24) by the
+// trapezoid rule, which converges geometrically for this entire integrand.
+// Replaces the FP32 A&S 9.8.7/9.8.8 fits of tan_tanadj/main_tan_tanadj.cu:57-89.
+// ---------------------------------------------------------------------------
+double k1_series(double x)
+{
+    const double euler = 0.57721566490153286061;
+    const double y = 0.25 * x * x;
+    double term = 1.0, psi1 = -euler, psi2 = 1.0 - euler, si = 0.0, s2 = 0.0;
+    for (int k = 0; k < 80; ++k) {
+        si += term;
+        s2 += (psi1 + psi2) * term;
+        term *= y / ((k + 1.0) * (k + 2.0));
+        psi1 += 1.0 / (k + 1.0);
+        psi2 += 1.0 / (k + 2.0);
+        if (term < 1e-20 * si) break;
+    }
+    const double i1 = 0.5 * x * si;
+    return 1.0 / x + std::log(0.5 * x) * i1 - 0.25 * x * s2;
+}
+
+double k1_integral(double x)
+{
+    const double h = 0.0625;
+    double acc = 0.5 * std::exp(-x);
+    for (int k = 1; k < 20000; ++k) {
+        const double c = std::cosh(k * h);
+        const double v = std::exp(-x * c) * c;
+        acc += v;
+        if (v < 1e-20 * acc) break;
+    }
+    return h * acc;
+}
+
+double bessel_k1(double x) { return x <= 2.0 ? k1_series(x) : k1_integral(x); }
+
+// SRSRK Green's tensor in FP64 (training/main_cd_training.cu:63-77,
+// tan_tanadj/main_tan_tanadj.cu:93-116): K[i*kw + j], i = x (column) index.
+void srsrk_kernel(int kw, double sigma, double dx, double cw, std::vector<double>& Kx,
+                  std::vector<double>& Ky)
+{
+    const double gamma = 1.0 / sigma, Rw = cw * sigma, coeff = gamma / (2.0 * M_PI);
+    const double eps = 1e-6 * dx;
+    const int c = kw / 2;
+    Kx.assign((size_t)kw * kw, 0.0);
+    Ky.assign((size_t)kw * kw, 0.0);
+    for (int i = 0; i < kw; ++i)
+        for (int j = 0; j < kw; ++j) {
+            const double x = (i - c) * dx, y = (j - c) * dx;
+            const double r = std::sqrt(x * x + y * y);
+            if (r < eps || r > Rw) continue;
+            const double t = r / Rw;
+            const double win = 0.42 + 0.5 * std::cos(M_PI * t) + 0.08 * std::cos(2.0 * M_PI * t);
+            const double base = coeff * bessel_k1(gamma * r) * win;
+            Kx[(size_t)i * kw + j] = base * (x / r);
+            Ky[(size_t)i * kw + j] = base * (y / r);
+        }
+}
+
+// 2-D DFT of a kw x kh col-major kernel centred at offset 0 (the reference's
+// pad + fftshift, srk_forward.cuh:162-187), in double, written in the
+// reference spectrum layout out[k_x*(M/2+1) + k_y].  Separable evaluation.
+void kernel_dft(const std::vector<double>& G, int kw, int kh, int M, std::vector<float2>& out)
+{
+    const int H = M / 2 + 1, cx = kw / 2, cy = kh / 2;
+    std::vector<std::complex<double>> tw(M);
+    for (int n = 0; n < M; ++n) tw[n] = std::polar(1.0, -2.0 * M_PI * n / M);
+    auto wmod = [M](long long v) { return (int)(((v % M) + M) % M); };
+    std::vector<std::complex<double>> T((size_t)kw * H);
+    for (int i = 0; i < kw; ++i)
+        for (int ky = 0; ky < H; ++ky) {
+            std::complex<double> acc = 0.0;
+            for (int j = 0; j < kh; ++j) {
+                const double g = G[(size_t)i * kh + j];
+                if (g != 0.0) acc += g * tw[wmod((long long)ky * (j - cy))];
+            }
+            T[(size_t)i * H + ky] = acc;
+        }
+    out.assign((size_t)M * H, make_float2(0.f, 0.f));
+    const int nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (int p = 0; p < nth; ++p)
+        th.emplace_back([&, p]() {
+            for (int kx = p; kx < M; kx += nth)
+                for (int ky = 0; ky < H; ++ky) {
+                    std::complex<double> acc = 0.0;
+                    for (int i = 0; i < kw; ++i)
+                        acc += tw[wmod((long long)kx * (i - cx))] * T[(size_t)i * H + ky];
+                    out[(size_t)kx * H + ky] = make_float2((float)acc.real(), (float)acc.imag());
+                }
+        });
+    for (auto& t : th) t.join();
+}
+
+struct SpecKey {
+    int kw;
+    double sigma, dx, cw;
+    int M;
+    bool operator<(const SpecKey& o) const
+    {
+        return std::tie(kw, sigma, dx, cw, M) < std::tie(o.kw, o.sigma, o.dx, o.cw, o.M);
+    }
+};
+struct SpecVal {
+    std::vector<float2> gx, gy;
+    std::vector<float> kx, ky;
+};
+std::mutex g_spec_mu;
+std::map<SpecKey, SpecVal> g_spec_cache;
+
+// ---------------------------------------------------------------------------
+// tile tables
+// ---------------------------------------------------------------------------
+// Forward: outputs cover the valid region [e, N-e)^2.  N <= M: one window at
+// the origin (circular wrap cannot reach the valid region because e >= KW/2).
+// Otherwise windows of stride M - 2e at p0 = t*S (the reference's tiling,
+// srk_forward.cuh:378-421).
+bool fwd_tiles(int N, int e, int M, std::vector<srk::Tile>& t, int& S2)
+{
+    const int W = N - 2 * e;
+    t.clear();
+    if (N <= M) {
+        t.push_back({0, 0, 0, e, N - e, e, N - e, 0});
+        S2 = W;
+        return true;
+    }
+    const int S = M - 2 * e;
+    if (S < 1) return false;
+    const int T = (W + S - 1) / S;
+    for (int ty = 0; ty < T; ++ty)
+        for (int tx = 0; tx < T; ++tx) {
+            const int ox0 = e + tx * S, oy0 = e + ty * S;
+            t.push_back({0, tx * S, ty * S, ox0, std::min(ox0 + S, N - e), oy0,
+                         std::min(oy0 + S, N - e), 0});
+        }
+    S2 = S;
+    return true;
+}
+
+// Adjoint: outputs cover all of [0, N)^2 (gather form, no overlap-add).
+// N <= M: one window at the origin; the sources vanish outside [e, N-e), so
+// wrapped taps read zeros.  Otherwise stride S = M - 2e - 2 with windows at
+// t*S - e - 1, so u is exact one pixel beyond each output tile (for the
+// stencil) and lambda_g is exact on the tile.
+bool adj_tiles(int N, int e, int M, std::vector<srk::Tile>& t, int& S2)
+{
+    t.clear();
+    if (N <= M) {
+        t.push_back({0, 0, 0, 0, N, 0, N, 0});
+        S2 = N + 2;
+        return true;
+    }
+    const int S = M - 2 * e - 2;
+    if (S < 1) return false;
+    const int T = (N + S - 1) / S;
+    for (int ty = 0; ty < T; ++ty)
+        for (int tx = 0; tx < T; ++tx) {
+            const int ox0 = tx * S, oy0 = ty * S;
+            t.push_back({0, ox0 - e - 1, oy0 - e - 1, ox0, std::min(ox0 + S, N), oy0,
+                         std::min(oy0 + S, N), 0});
+        }
+    S2 = S + 2;
+    return true;
+}
+
+}  // namespace
+
+struct srk_workspace_s {
+    int N = 0, W = 0, e = 0, M = 0, B_max = 0;
+    int chunk_override = 0;
+    srk::Tile* d_ftiles = nullptr;
+    srk::Tile* d_atiles = nullptr;
+    int nft = 0, nat = 0, S2f = 0, S2a = 0;
+    float2* X1 = nullptr;
+    float2* X2 = nullptr;
+    size_t x1_bytes = 0, x2_bytes = 0;
+    float2* GT = nullptr;
+    float2* tw = nullptr;
+    size_t total = 0;
+};
+
+namespace {
+
+void ws_free(srk_workspace_s* w)
+{
+    if (!w) return;
+    hipFree(w->d_ftiles);
+    hipFree(w->d_atiles);
+    hipFree(w->X1);
+    hipFree(w->X2);
+    hipFree(w->GT);
+    hipFree(w->tw);
+    delete w;
+}
+
+int chunk_for(const srk_workspace_s* w, int np1, int np2, int S2, int ntiles)
+{
+    const int HP = srk::spec_pitch(w->M);
+    const size_t per1 = (size_t)np1 * w->M * HP * sizeof(float2);
+    const size_t per2 = (size_t)np2 * S2 * HP * sizeof(float2);
+    const size_t cap = std::min(w->x1_bytes / per1, w->x2_bytes / per2);
+    size_t c = w->chunk_override > 0 ? std::min<size_t>(w->chunk_override, cap) : cap;
+    c = std::max<size_t>(1, std::min<size_t>(c, (size_t)ntiles));
+    return (int)c;
+}
+
+srk::EngineArgs base_args(const srk_workspace_s* w, float dx)
+{
+    (void)dx;
+    srk::EngineArgs a{};
+    a.N = w->N;
+    a.e = w->e;
+    a.W = w->W;
+    a.img_stride = (long long)w->N * w->N;
+    a.w_stride = (long long)w->W * w->W;
+    a.GT = w->GT;
+    a.tw = w->tw;
+    a.X1 = w->X1;
+    a.X2 = w->X2;
+    return a;
+}
+
+int check_ws(const srk_workspace_s* w, int N, int W, int e, int B)
+{
+    if (!w) return SRK_ERR_NULL_PTR;
+    if (W != N - 2 * e || W <= 0 || e < 1) return SRK_ERR_BAD_DIMS;
+    if (N != w->N || W != w->W || e != w->e) return SRK_ERR_WORKSPACE;
+    if (B < 1 || B > w->B_max) return SRK_ERR_BATCH;
+    return SRK_OK;
+}
+
+int check_f(const float* p)
+{
+    if (!p) return SRK_ERR_NULL_PTR;
+    if (!aligned(p, sizeof(float))) return SRK_ERR_MISALIGNED;
+    return SRK_OK;
+}
+
+int check_spec(const void* gx, const void* gy)
+{
+    if (!gx || !gy) return SRK_ERR_NULL_PTR;
+    if (!aligned((const char*)gx, 8) || !aligned((const char*)gy, 8)) return SRK_ERR_MISALIGNED;
+    return SRK_OK;
+}
+
+#define SRK_TRY(expr)                 \
+    do {                              \
+        int _s = (expr);              \
+        if (_s != SRK_OK) return _s;  \
+    } while (0)
+
+int import_spectra(srk_workspace_s* w, const void* gx, const void* gy, float dx, hipStream_t s)
+{
+    const float scale = (float)((double)dx * dx / ((double)w->M * w->M));
+    return hip_status(srk::engine_import_spectra(w->M, (const float2*)gx, (const float2*)gy,
+                                                 scale, w->GT, s));
+}
+
+int run_fwd(srk_workspace_s* w, srk::Op op, srk::EngineArgs a, int B, hipStream_t s)
+{
+    const int nf = op == srk::Op::Tangent ? 2 : 1;
+    a.tiles = w->d_ftiles;
+    a.np1 = nf;
+    a.np2 = 2 * nf;
+    a.S2 = w->S2f;
+    a.halo2 = 0;
+    const int ntiles = B * w->nft;
+    const int chunk = chunk_for(w, a.np1, a.np2, a.S2, ntiles);
+    return hip_status(srk::engine_run(op, w->M, a, ntiles, chunk, s));
+}
+
+int run_adj(srk_workspace_s* w, srk::Op op, srk::EngineArgs a, int B, hipStream_t s)
+{
+    switch (op) {
+    case srk::Op::Adjoint: a.np1 = 3; a.np2 = 3; break;
+    case srk::Op::AdjointG: a.np1 = 2; a.np2 = 1; break;
+    case srk::Op::TangentAdjoint: a.np1 = 4; a.np2 = 5; break;
+    default: return SRK_ERR_INVALID_ARG;
+    }
+    a.tiles = w->d_atiles;
+    a.S2 = w->S2a;
+    a.halo2 = 1;
+    const int ntiles = B * w->nat;
+    const int chunk = chunk_for(w, a.np1, a.np2, a.S2, ntiles);
+    return hip_status(srk::engine_run(op, w->M, a, ntiles, chunk, s));
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* srkGetErrorString(int status)
+{
+    switch (status) {
+    case SRK_OK: return "success";
+    case SRK_ERR_INVALID_ARG: return "invalid argument";
+    case SRK_ERR_BAD_DIMS: return "inconsistent dimensions (need W = N - 2*erosion > 0, erosion >= 1)";
+    case SRK_ERR_NULL_PTR: return "null pointer";
+    case SRK_ERR_MISALIGNED: return "misaligned pointer";
+    case SRK_ERR_UNSUPPORTED_SIZE: return "unsupported tile size M (256, 512, 1024, 2048; M > 2*erosion + 2)";
+    case SRK_ERR_WORKSPACE: return "workspace was created for different N / W / erosion / M";
+    case SRK_ERR_BATCH: return "batch size outside [1, B_max]";
+    case SRK_ERR_HIP: return "HIP runtime error";
+    case SRK_ERR_KERNEL: return "invalid kernel parameters";
+    default: return "unknown status";
+    }
+}
+
+int srkLastHipError(void) { return g_last_hip; }
+int srkGetVersion(void) { return 100; }
+
+int srkSynchronize(hipStream_t stream) { return hip_status(hipStreamSynchronize(stream)); }
+
+int srkPrecomputeKernelFreq(const float* h_G, int kw, int kh, int M, void* d_G_freq,
+                            hipStream_t stream)
+{
+    if (!h_G || !d_G_freq) return SRK_ERR_NULL_PTR;
+    if (kw < 1 || kh < 1 || kw % 2 == 0 || kh % 2 == 0 || kw > M || kh > M) return SRK_ERR_KERNEL;
+    if (M < 2 || (M & (M - 1))) return SRK_ERR_UNSUPPORTED_SIZE;
+    std::vector<double> G((size_t)kw * kh);
+    for (size_t i = 0; i < G.size(); ++i) G[i] = h_G[i];
+    std::vector<float2> out;
+    kernel_dft(G, kw, kh, M, out);
+    SRK_TRY(hip_status(hipMemcpyAsync(d_G_freq, out.data(), out.size() * sizeof(float2),
+                                      hipMemcpyHostToDevice, stream)));
+    return hip_status(hipStreamSynchronize(stream));
+}
+
+int srkKernelSpectraSRSRK(int kw, double sigma, double dx, double cw, int M, void* d_Gx_freq,
+                          void* d_Gy_freq, float* h_Gx, float* h_Gy, hipStream_t stream)
+{
+    if (kw < 3 || kw % 2 == 0 || !(sigma > 0) || !(dx > 0) || !(cw > 0)) return SRK_ERR_KERNEL;
+    if (M < 2 || (M & (M - 1)) || kw > M) return SRK_ERR_UNSUPPORTED_SIZE;
+    const SpecKey key{kw, sigma, dx, cw, M};
+    SpecVal* val = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_spec_mu);
+        auto it = g_spec_cache.find(key);
+        if (it == g_spec_cache.end()) {
+            std::vector<double> Kx, Ky;
+            srsrk_kernel(kw, sigma, dx, cw, Kx, Ky);
+            SpecVal v;
+            kernel_dft(Kx, kw, kw, M, v.gx);
+            kernel_dft(Ky, kw, kw, M, v.gy);
+            v.kx.assign(Kx.begin(), Kx.end());
+            v.ky.assign(Ky.begin(), Ky.end());
+            it = g_spec_cache.emplace(key, std::move(v)).first;
+        }
+        val = &it->second;
+    }
+    if (h_Gx) std::copy(val->kx.begin(), val->kx.end(), h_Gx);
+    if (h_Gy) std::copy(val->ky.begin(), val->ky.end(), h_Gy);
+    if (d_Gx_freq)
+        SRK_TRY(hip_status(hipMemcpyAsync(d_Gx_freq, val->gx.data(), val->gx.size() * sizeof(float2),
+                                          hipMemcpyHostToDevice, stream)));
+    if (d_Gy_freq)
+        SRK_TRY(hip_status(hipMemcpyAsync(d_Gy_freq, val->gy.data(), val->gy.size() * sizeof(float2),
+                                          hipMemcpyHostToDevice, stream)));
+    if (!d_Gx_freq && !d_Gy_freq) return SRK_OK;  // host kernels only: no device work
+    return hip_status(hipStreamSynchronize(stream));
+}
+
+int srkWorkspaceCreate(int N, int W, int erosion, int M, int B_max, size_t scratch_mb,
+                       srk_workspace_t* out)
+{
+    if (!out) return SRK_ERR_NULL_PTR;
+    *out = nullptr;
+    if (W != N - 2 * erosion || W <= 0 || erosion < 1) return SRK_ERR_BAD_DIMS;
+    if (B_max < 1) return SRK_ERR_BATCH;
+    if (!supported_M(M) || M <= 2 * erosion + 2) return SRK_ERR_UNSUPPORTED_SIZE;
+    std::vector<srk::Tile> ft, at;
+    int S2f = 0, S2a = 0;
+    if (!fwd_tiles(N, erosion, M, ft, S2f) || !adj_tiles(N, erosion, M, at, S2a))
+        return SRK_ERR_UNSUPPORTED_SIZE;
+    auto* w = new srk_workspace_s();
+    w->N = N; w->W = W; w->e = erosion; w->M = M; w->B_max = B_max;
+    w->nft = (int)ft.size(); w->nat = (int)at.size(); w->S2f = S2f; w->S2a = S2a;
+    // replicate the per-image tables over the batch
+    std::vector<srk::Tile> fb((size_t)B_max * ft.size()), ab((size_t)B_max * at.size());
+    for (int b = 0; b < B_max; ++b) {
+        for (size_t i = 0; i < ft.size(); ++i) { fb[b * ft.size() + i] = ft[i]; fb[b * ft.size() + i].b = b; }
+        for (size_t i = 0; i < at.size(); ++i) { ab[b * at.size() + i] = at[i]; ab[b * at.size() + i].b = b; }
+    }
+    const int HP = srk::spec_pitch(M);
+    const size_t x1_per = (size_t)4 * M * HP * sizeof(float2);
+    const size_t x2_per = (size_t)std::max(4 * S2f, 5 * S2a) * HP * sizeof(float2);
+    const size_t budget = (scratch_mb ? scratch_mb : 256) << 20;
+    size_t cmax = std::max<size_t>(1, budget / (x1_per + x2_per));
+    cmax = std::min<size_t>(cmax, (size_t)B_max * std::max(w->nft, w->nat));
+    w->x1_bytes = cmax * x1_per;
+    w->x2_bytes = cmax * x2_per;
+    std::vector<float2> tw(M);
+    for (int k = 0; k < M; ++k) {
+        const double a = -2.0 * M_PI * k / M;
+        tw[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    int st = SRK_OK;
+    auto alloc = [&](void** p, size_t bytes) {
+        if (st != SRK_OK) return;
+        st = hip_status(hipMalloc(p, bytes));
+        if (st == SRK_OK) w->total += bytes;
+    };
+    alloc((void**)&w->d_ftiles, fb.size() * sizeof(srk::Tile));
+    alloc((void**)&w->d_atiles, ab.size() * sizeof(srk::Tile));
+    alloc((void**)&w->X1, w->x1_bytes);
+    alloc((void**)&w->X2, w->x2_bytes);
+    alloc((void**)&w->GT, (size_t)2 * (M / 2 + 1) * M * sizeof(float2));
+    alloc((void**)&w->tw, (size_t)M * sizeof(float2));
+    if (st == SRK_OK)
+        st = hip_status(hipMemcpy(w->d_ftiles, fb.data(), fb.size() * sizeof(srk::Tile), hipMemcpyHostToDevice));
+    if (st == SRK_OK)
+        st = hip_status(hipMemcpy(w->d_atiles, ab.data(), ab.size() * sizeof(srk::Tile), hipMemcpyHostToDevice));
+    if (st == SRK_OK)
+        st = hip_status(hipMemcpy(w->tw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
+    if (st != SRK_OK) {
+        ws_free(w);
+        return st;
+    }
+    if (const char* env = std::getenv("SRK_CHUNK_TILES")) w->chunk_override = std::atoi(env);
+    *out = w;
+    return SRK_OK;
+}
+
+int srkWorkspaceDestroy(srk_workspace_t w)
+{
+    if (!w) return SRK_ERR_NULL_PTR;
+    ws_free(w);
+    return SRK_OK;
+}
+
+size_t srkWorkspaceBytes(srk_workspace_t w) { return w ? w->total : 0; }
+
+int srkWorkspaceSetChunk(srk_workspace_t w, int tiles)
+{
+    if (!w) return SRK_ERR_NULL_PTR;
+    if (tiles < 0) return SRK_ERR_INVALID_ARG;
+    w->chunk_override = tiles;
+    return SRK_OK;
+}
+
+int srkWorkspaceInfo(srk_workspace_t w, int* M, int* fwd_tiles, int* adj_tiles, int* chunk_fwd,
+                     int* chunk_adj)
+{
+    if (!w) return SRK_ERR_NULL_PTR;
+    if (M) *M = w->M;
+    if (fwd_tiles) *fwd_tiles = w->nft;
+    if (adj_tiles) *adj_tiles = w->nat;
+    if (chunk_fwd) *chunk_fwd = chunk_for(w, 1, 2, w->S2f, w->B_max * w->nft);
+    if (chunk_adj) *chunk_adj = chunk_for(w, 3, 3, w->S2a, w->B_max * w->nat);
+    return SRK_OK;
+}
+
+int srkForwardBatched(const float* E, const float* I, const void* Gx, const void* Gy, int N, int W,
+                      int erosion, float dx, int B, hipStream_t stream, float* R,
+                      srk_workspace_t w)
+{
+    SRK_TRY(check_ws(w, N, W, erosion, B));
+    SRK_TRY(check_f(E)); SRK_TRY(check_f(I)); SRK_TRY(check_f(R)); SRK_TRY(check_spec(Gx, Gy));
+    SRK_TRY(import_spectra(w, Gx, Gy, dx, stream));
+    srk::EngineArgs a = base_args(w, dx);
+    a.g0 = E; a.f0 = I; a.out0 = R;
+    return run_fwd(w, srk::Op::Forward, a, B, stream);
+}
+
+int srkTrainingForward(const float* E, const float* I, const void* Gx, const void* Gy, int N,
+                       int W, int erosion, float dx, int B, hipStream_t stream, float* R,
+                       srk_workspace_t w)
+{
+    if (w && w->M != N) return SRK_ERR_WORKSPACE;  // spectra are at size N
+    return srkForwardBatched(E, I, Gx, Gy, N, W, erosion, dx, B, stream, R, w);
+}
+
+int srkForwardOaS(const float* E, const float* I, const void* Gx, const void* Gy, int N, int W,
+                  int erosion, float dx, hipStream_t stream, float* R, srk_workspace_t w,
+                  float* ux, float* uy)
+{
+    SRK_TRY(check_ws(w, N, W, erosion, 1));
+    SRK_TRY(check_f(E)); SRK_TRY(check_f(I)); SRK_TRY(check_f(R)); SRK_TRY(check_spec(Gx, Gy));
+    if ((ux == nullptr) != (uy == nullptr)) return SRK_ERR_NULL_PTR;
+    if (ux) { SRK_TRY(check_f(ux)); SRK_TRY(check_f(uy)); }
+    SRK_TRY(import_spectra(w, Gx, Gy, dx, stream));
+    srk::EngineArgs a = base_args(w, dx);
+    a.g0 = E; a.f0 = I; a.out0 = R;
+    a.ux_out = ux; a.uy_out = uy;
+    return run_fwd(w, srk::Op::Forward, a, 1, stream);
+}
+
+int adjointSRK(const float* v, const float* ux, const float* uy, const float* I, const void* Gx,
+               const void* Gy, int N, int W, int erosion, float dx, hipStream_t stream,
+               float* dLdI, float* dLdE, srk_workspace_t w)
+{
+    SRK_TRY(check_ws(w, N, W, erosion, 1));
+    SRK_TRY(check_f(v)); SRK_TRY(check_f(ux)); SRK_TRY(check_f(uy)); SRK_TRY(check_f(I));
+    SRK_TRY(check_f(dLdI)); SRK_TRY(check_f(dLdE)); SRK_TRY(check_spec(Gx, Gy));
+    SRK_TRY(hip_status(srk::engine_adjoint_f_from_u(v, ux, uy, dLdI, N, W, erosion, 1,
+                                                    (long long)N * N, (long long)W * W, stream)));
+    SRK_TRY(import_spectra(w, Gx, Gy, dx, stream));
+    srk::EngineArgs a = base_args(w, dx);
+    a.f0 = I; a.v0 = v; a.out1 = dLdE;
+    return run_adj(w, srk::Op::AdjointG, a, 1, stream);
+}
+
+int srkAdjointBatched(const float* v, const float* E, const float* I, const void* Gx,
+                      const void* Gy, int N, int W, int erosion, float dx, int B,
+                      hipStream_t stream, float* dLdI, float* dLdE, srk_workspace_t w)
+{
+    SRK_TRY(check_ws(w, N, W, erosion, B));
+    SRK_TRY(check_f(v)); SRK_TRY(check_f(E)); SRK_TRY(check_f(I));
+    SRK_TRY(check_f(dLdI)); SRK_TRY(check_f(dLdE)); SRK_TRY(check_spec(Gx, Gy));
+    SRK_TRY(import_spectra(w, Gx, Gy, dx, stream));
+    srk::EngineArgs a = base_args(w, dx);
+    a.g0 = E; a.f0 = I; a.v0 = v; a.out0 = dLdI; a.out1 = dLdE;
+    return run_adj(w, srk::Op::Adjoint, a, B, stream);
+}
+
+int srkTangentBatched(const float* f, const float* g, const float* df, const float* dg,
+                      const void* Gx, const void* Gy, int N, int W, int erosion, float dx, int B,
+                      hipStream_t stream, float* dh, srk_workspace_t w)
+{
+    SRK_TRY(check_ws(w, N, W, erosion, B));
+    SRK_TRY(check_f(f)); SRK_TRY(check_f(g)); SRK_TRY(check_f(df)); SRK_TRY(check_f(dg));
+    SRK_TRY(check_f(dh)); SRK_TRY(check_spec(Gx, Gy));
+    SRK_TRY(import_spectra(w, Gx, Gy, dx, stream));
+    srk::EngineArgs a = base_args(w, dx);
+    a.g0 = g; a.g1 = dg; a.f0 = f; a.f1 = df; a.out0 = dh;
+    return run_fwd(w, srk::Op::Tangent, a, B, stream);
+}
+
+int shrinkageTangentOaS(const float* f, const float* g, const float* df, const float* dg,
+                        const void* Gx, const void* Gy, int N, int W, int erosion, float dx,
+                        hipStream_t stream, float* dh, srk_workspace_t w)
+{
+    return srkTangentBatched(f, g, df, dg, Gx, Gy, N, W, erosion, dx, 1, stream, dh, w);
+}
+
+int srkTangentAdjointBatched(const float* f, const float* g, const float* df, const float* dg,
+                             const float* lam_h, const float* dlam_h, const void* Gx,
+                             const void* Gy, int N, int W, int erosion, float dx, int B,
+                             hipStream_t stream, float* dLam_f, float* dLam_g,
+                             srk_workspace_t w)
+{
+    SRK_TRY(check_ws(w, N, W, erosion, B));
+    SRK_TRY(check_f(f)); SRK_TRY(check_f(g)); SRK_TRY(check_f(df)); SRK_TRY(check_f(dg));
+    SRK_TRY(check_f(lam_h)); SRK_TRY(check_f(dlam_h));
+    SRK_TRY(check_f(dLam_f)); SRK_TRY(check_f(dLam_g)); SRK_TRY(check_spec(Gx, Gy));
+    SRK_TRY(import_spectra(w, Gx, Gy, dx, stream));
+    srk::EngineArgs a = base_args(w, dx);
+    // w_d = u_d(g) dlam_h + u_d(dg) lam_h ;  a_d = dlam_h d_d f + lam_h d_d df
+    a.g0 = g; a.g1 = dg; a.f0 = f; a.f1 = df; a.v0 = dlam_h; a.v1 = lam_h;
+    a.out0 = dLam_f; a.out1 = dLam_g;
+    return run_adj(w, srk::Op::TangentAdjoint, a, B, stream);
+}
+
+int shrinkageTangentAdjointOaS(const float* f, const float* g, const float* df,
+                               const float* dg, const float* lam_h, const float* dlam_h,
+                               const void* Gx, const void* Gy, int N, int W, int erosion,
+                               float dx, hipStream_t stream, float* dLam_f, float* dLam_g,
+                               srk_workspace_t w)
+{
+    return srkTangentAdjointBatched(f, g, df, dg, lam_h, dlam_h, Gx, Gy, N, W, erosion, dx, 1,
+                                    stream, dLam_f, dLam_g, w);
+}
+
+}  // extern "C"

@@ -1,0 +1,649 @@
+// srk_engine.hip -- overlap-and-save tile engine for the SRK shrinkage operator
+// family on gfx950.  See srk_engine.hpp for the pass structure.
+//
+// Reference behaviour restated (file:line in the reference tree):
+//   forward      srk_forward.cuh:354-501        (true convolution, dx^2/M^2 scale,
+//                                                valid-region crop, central differences)
+//   lambda_f     adjoint/srk_adjoint.cuh:25-75   (transpose of stencil + combine)
+//   lambda_g     adjoint/main_adjoint.cu:206-246 (correlation = product with conj(Ghat))
+//   tangent      tan_tanadj/srk_tangent.cuh:28-151
+//   tangent-adj  tan_tanadj/srk_tangent_adjoint.cuh:100-159
+#include "srk_engine.hpp"
+#include "srk_fft.hpp"
+
+#include <algorithm>
+
+namespace srk {
+
+namespace {
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+__device__ __forceinline__ int wave_id() { return threadIdx.x / kWave; }
+
+// Unpack the transform Z of two real sequences (p + i q) into the half spectra
+// P(k) = (Z(k) + conj Z(M-k))/2 and Q(k) = (Z(k) - conj Z(M-k))/(2i), k <= M/2.
+template <int M>
+__device__ __forceinline__ void unpack_store(const float2* line, float2* P, float2* Q, int lane)
+{
+    for (int k = lane; k <= M / 2; k += kWave) {
+        const float2 z = line[k];
+        const float2 zm = line[(M - k) & (M - 1)];
+        P[k] = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
+        Q[k] = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
+    }
+}
+
+template <int M>
+__device__ __forceinline__ void zero_store(float2* P, int lane)
+{
+    for (int k = lane; k <= M / 2; k += kWave) P[k] = make_float2(0.f, 0.f);
+}
+
+// Inverse transform of two Hermitian half spectra P, Q (either may be null for
+// zero) packed as Z = P + iQ; afterwards line[y] = p(y) + i q(y).  The DC and
+// Nyquist bins are taken as real, as a real inverse transform does.
+template <int M>
+__device__ __forceinline__ void ifft_pair(float2* line, const float2* P, const float2* Q,
+                                          const float2* tw, int lane)
+{
+    constexpr int V = M / kWave;
+    float2 v[V];
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+        const int k = lane + kWave * i;
+        const int kk = k <= M / 2 ? k : M - k;
+        float2 p = P ? P[kk] : make_float2(0.f, 0.f);
+        float2 q = Q ? Q[kk] : make_float2(0.f, 0.f);
+        if (kk == 0 || kk == M / 2) {
+            p.y = 0.f;
+            q.y = 0.f;
+        }
+        v[i] = k <= M / 2 ? make_float2(p.x - q.y, p.y + q.x) : make_float2(p.x + q.y, q.x - p.y);
+    }
+    fft_from_regs<M, +1>(line, v, tw, lane);
+}
+
+__device__ __forceinline__ float grad_x(const float* f, int N, int r, int c)
+{
+    const float right = (c + 1 < N) ? f[(long long)(c + 1) * N + r] : 0.f;
+    const float left = (c - 1 >= 0) ? f[(long long)(c - 1) * N + r] : 0.f;
+    return 0.5f * (right - left);
+}
+__device__ __forceinline__ float grad_y(const float* f, int N, int r, int c)
+{
+    const float down = (r + 1 < N) ? f[(long long)c * N + r + 1] : 0.f;
+    const float up = (r - 1 >= 0) ? f[(long long)c * N + r - 1] : 0.f;
+    return 0.5f * (down - up);
+}
+
+// Cotangent v on the valid region, zero outside (global coordinates).
+__device__ __forceinline__ float cot_at(const float* v, int N, int e, int W, int c, int r)
+{
+    if (c < e || c >= N - e || r < e || r >= N - e) return 0.f;
+    return v[(long long)(c - e) * W + (r - e)];
+}
+
+// ---------------------------------------------------------------------------
+// P1: real FFT along y of the convolved fields; one wavefront per column pair.
+// ---------------------------------------------------------------------------
+template <int M>
+__global__ __launch_bounds__(256) void k_p1_cols(EngineArgs a, int nf)
+{
+    constexpr int V = M / kWave;
+    constexpr int HP = spec_pitch(M);
+    __shared__ float2 lds[4][M];
+    const int lane = lane_id(), w = wave_id();
+    const int slot = blockIdx.y;
+    const Tile t = a.tiles[slot];
+    const int N = a.N;
+    const int xa = (blockIdx.x * 4 + w) * 2;
+    const int ca = t.p0x + xa, cb = ca + 1;
+    const bool va = ca >= 0 && ca < N, vb = cb >= 0 && cb < N;
+    float2* line = lds[w];
+    for (int fi = 0; fi < nf; ++fi) {
+        float2* outA = a.X1 + (((long long)slot * a.np1 + fi) * M + xa) * HP;
+        float2* outB = outA + HP;
+        if (!va && !vb) {
+            zero_store<M>(outA, lane);
+            zero_store<M>(outB, lane);
+            continue;
+        }
+        const float* src = (fi == 0 ? a.g0 : a.g1) + (long long)t.b * a.img_stride;
+        const float* colA = src + (long long)ca * N;
+        const float* colB = src + (long long)cb * N;
+        float2 v[V];
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            const int row = t.p0y + lane + kWave * i;
+            const bool rv = row >= 0 && row < N;
+            v[i].x = (va && rv) ? colA[row] : 0.f;
+            v[i].y = (vb && rv) ? colB[row] : 0.f;
+        }
+        fft_from_regs<M, -1>(line, v, a.tw, lane);
+        unpack_store<M>(line, outA, outB, lane);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// P1A: adjoint source a_d = v0 * d_d f0 (+ v1 * d_d f1) on the valid region,
+// transformed along y as a_x + i a_y; one wavefront per column.
+// ---------------------------------------------------------------------------
+template <int M>
+__global__ __launch_bounds__(256) void k_p1_adj_src(EngineArgs a, int nf, int plane0)
+{
+    constexpr int V = M / kWave;
+    constexpr int HP = spec_pitch(M);
+    __shared__ float2 lds[4][M];
+    const int lane = lane_id(), w = wave_id();
+    const int slot = blockIdx.y;
+    const Tile t = a.tiles[slot];
+    const int N = a.N, e = a.e, W = a.W;
+    const int x = blockIdx.x * 4 + w;
+    const int gx = t.p0x + x;
+    float2* outX = a.X1 + (((long long)slot * a.np1 + plane0) * M + x) * HP;
+    float2* outY = a.X1 + (((long long)slot * a.np1 + plane0 + 1) * M + x) * HP;
+    if (gx < e || gx >= N - e) {
+        zero_store<M>(outX, lane);
+        zero_store<M>(outY, lane);
+        return;
+    }
+    const float* f0 = a.f0 + (long long)t.b * a.img_stride;
+    const float* v0 = a.v0 + (long long)t.b * a.w_stride;
+    const float* f1 = nf > 1 ? a.f1 + (long long)t.b * a.img_stride : nullptr;
+    const float* v1 = nf > 1 ? a.v1 + (long long)t.b * a.w_stride : nullptr;
+    float2 v[V];
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+        const int gy = t.p0y + lane + kWave * i;
+        float ax = 0.f, ay = 0.f;
+        if (gy >= e && gy < N - e) {
+            const float c0 = v0[(long long)(gx - e) * W + (gy - e)];
+            ax = c0 * grad_x(f0, N, gy, gx);
+            ay = c0 * grad_y(f0, N, gy, gx);
+            if (nf > 1) {
+                const float c1 = v1[(long long)(gx - e) * W + (gy - e)];
+                ax += c1 * grad_x(f1, N, gy, gx);
+                ay += c1 * grad_y(f1, N, gy, gx);
+            }
+        }
+        v[i] = make_float2(ax, ay);
+    }
+    float2* line = lds[w];
+    fft_from_regs<M, -1>(line, v, a.tw, lane);
+    unpack_store<M>(line, outX, outY, lane);
+}
+
+// Shared helpers of the P2 kernels: transposed load of RY k_y bands of one X1
+// plane into LDS lines (line r holds band k0 + r along x), and the transposed
+// store of LDS lines into an X2 plane for tile-local columns [lo, lo + cnt).
+template <int M, int RY, int LP>
+__device__ __forceinline__ void load_bands(float2* L, const float2* plane, int k0, int nrows)
+{
+    constexpr int HP = spec_pitch(M);
+    for (int idx = threadIdx.x; idx < M * RY; idx += blockDim.x) {
+        const int x = idx / RY, r = idx - x * RY;
+        L[r * LP + x] = r < nrows ? plane[(long long)x * HP + k0 + r] : make_float2(0.f, 0.f);
+    }
+}
+
+template <int M, int RY, int LP>
+__device__ __forceinline__ void store_bands(float2* plane, const float2* L, int k0, int nrows,
+                                            int lo, int cnt)
+{
+    constexpr int HP = spec_pitch(M);
+    for (int idx = threadIdx.x; idx < cnt * RY; idx += blockDim.x) {
+        const int xi = idx / RY, r = idx - xi * RY;
+        if (r < nrows) {
+            const int x = lo + xi;
+            plane[(long long)xi * HP + k0 + r] =
+                (x >= 0 && x < M) ? L[r * LP + x] : make_float2(0.f, 0.f);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// P2 conv: x-FFT of each convolved-field band, products with Gx and Gy,
+// inverse x-FFTs, crop; field fi -> X2 planes 2fi (u_x) and 2fi+1 (u_y).
+// ---------------------------------------------------------------------------
+template <int M, int RY, int NW>
+__global__ __launch_bounds__(NW * kWave) void k_p2_conv(EngineArgs a, int nf)
+{
+    constexpr int H = M / 2 + 1;
+    constexpr int HP = spec_pitch(M);
+    constexpr int LP = M + 1;
+    extern __shared__ float2 sm[];
+    float2* A = sm;
+    float2* Bs = sm + RY * LP;
+    const int lane = lane_id(), w = wave_id();
+    const int slot = blockIdx.y;
+    const Tile t = a.tiles[slot];
+    const int k0 = blockIdx.x * RY;
+    const int nrows = min(RY, H - k0);
+    const int lo = t.ox0 - t.p0x - a.halo2;
+    const int cnt = t.ox1 - t.ox0 + 2 * a.halo2;
+    for (int fi = 0; fi < nf; ++fi) {
+        const float2* src = a.X1 + ((long long)slot * a.np1 + fi) * M * HP;
+        load_bands<M, RY, LP>(A, src, k0, nrows);
+        __syncthreads();
+        for (int r = w; r < nrows; r += NW) {
+            float2* la = A + r * LP;
+            float2* lb = Bs + r * LP;
+            fft_in_lds<M, -1>(la, a.tw, lane);
+            const float2* gxr = a.GT + (long long)(k0 + r) * M;
+            const float2* gyr = a.GT + (long long)(H + k0 + r) * M;
+            for (int kx = lane; kx < M; kx += kWave) {
+                const float2 X = la[kx];
+                lb[kx] = cmul(X, gyr[kx]);
+                la[kx] = cmul(X, gxr[kx]);
+            }
+            fft_in_lds<M, +1>(la, a.tw, lane);
+            fft_in_lds<M, +1>(lb, a.tw, lane);
+        }
+        __syncthreads();
+        float2* dstA = a.X2 + ((long long)slot * a.np2 + 2 * fi) * a.S2 * HP;
+        float2* dstB = a.X2 + ((long long)slot * a.np2 + 2 * fi + 1) * a.S2 * HP;
+        store_bands<M, RY, LP>(dstA, A, k0, nrows, lo, cnt);
+        store_bands<M, RY, LP>(dstB, Bs, k0, nrows, lo, cnt);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// P2 corr: x-FFT of the adjoint sources, L = conj(Gx) A_x + conj(Gy) A_y,
+// inverse x-FFT, crop -> X2 plane planeOut.
+// ---------------------------------------------------------------------------
+template <int M, int RY, int NW>
+__global__ __launch_bounds__(NW * kWave) void k_p2_corr(EngineArgs a, int planeIn, int planeOut)
+{
+    constexpr int H = M / 2 + 1;
+    constexpr int HP = spec_pitch(M);
+    constexpr int LP = M + 1;
+    extern __shared__ float2 sm[];
+    float2* A = sm;
+    float2* Bs = sm + RY * LP;
+    const int lane = lane_id(), w = wave_id();
+    const int slot = blockIdx.y;
+    const Tile t = a.tiles[slot];
+    const int k0 = blockIdx.x * RY;
+    const int nrows = min(RY, H - k0);
+    const int lo = t.ox0 - t.p0x - a.halo2;
+    const int cnt = t.ox1 - t.ox0 + 2 * a.halo2;
+    load_bands<M, RY, LP>(A, a.X1 + ((long long)slot * a.np1 + planeIn) * M * HP, k0, nrows);
+    load_bands<M, RY, LP>(Bs, a.X1 + ((long long)slot * a.np1 + planeIn + 1) * M * HP, k0, nrows);
+    __syncthreads();
+    for (int r = w; r < nrows; r += NW) {
+        float2* la = A + r * LP;
+        float2* lb = Bs + r * LP;
+        fft_in_lds<M, -1>(la, a.tw, lane);
+        fft_in_lds<M, -1>(lb, a.tw, lane);
+        const float2* gxr = a.GT + (long long)(k0 + r) * M;
+        const float2* gyr = a.GT + (long long)(H + k0 + r) * M;
+        for (int kx = lane; kx < M; kx += kWave) {
+            const float2 p = cmulc(la[kx], gxr[kx]);
+            const float2 q = cmulc(lb[kx], gyr[kx]);
+            la[kx] = cadd(p, q);
+        }
+        fft_in_lds<M, +1>(la, a.tw, lane);
+    }
+    __syncthreads();
+    store_bands<M, RY, LP>(a.X2 + ((long long)slot * a.np2 + planeOut) * a.S2 * HP, A, k0, nrows,
+                           lo, cnt);
+}
+
+// ---------------------------------------------------------------------------
+// P3 forward / tangent: inverse y-FFT of (u_x + i u_y) per output column,
+// gradient stencil and combine, valid-region store.
+// ---------------------------------------------------------------------------
+template <int M>
+__global__ __launch_bounds__(256) void k_p3_fwd(EngineArgs a, int nf)
+{
+    constexpr int V = M / kWave;
+    constexpr int HP = spec_pitch(M);
+    __shared__ float2 lds[4][M];
+    const int lane = lane_id(), w = wave_id();
+    const int slot = blockIdx.y;
+    const Tile t = a.tiles[slot];
+    const int xi = blockIdx.x * 4 + w;
+    if (xi >= t.ox1 - t.ox0) return;
+    const int N = a.N, e = a.e, W = a.W;
+    const int gx = t.ox0 + xi;
+    float2* line = lds[w];
+    const float2* base = a.X2 + (long long)slot * a.np2 * a.S2 * HP + (long long)xi * HP;
+    const long long pstride = (long long)a.S2 * HP;
+    const float* f0 = a.f0 + (long long)t.b * a.img_stride;
+    float* out = a.out0 + (long long)t.b * a.w_stride + (long long)(gx - e) * W;
+
+    // u(g) for this column
+    ifft_pair<M>(line, base, base + pstride, a.tw, lane);
+    float2 u0[V];
+#pragma unroll
+    for (int j = 0; j < V; ++j) {
+        const int gy = t.oy0 + lane + kWave * j;
+        u0[j] = gy < t.oy1 ? line[gy - t.p0y] : make_float2(0.f, 0.f);
+    }
+    if (nf == 1) {
+#pragma unroll
+        for (int j = 0; j < V; ++j) {
+            const int gy = t.oy0 + lane + kWave * j;
+            if (gy < t.oy1)
+                out[gy - e] = u0[j].x * grad_x(f0, N, gy, gx) + u0[j].y * grad_y(f0, N, gy, gx);
+        }
+        if (a.ux_out) {
+            const long long o = (long long)t.b * a.w_stride + (long long)(gx - e) * W - e;
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                const int gy = t.oy0 + lane + kWave * j;
+                if (gy < t.oy1) {
+                    a.ux_out[o + gy] = u0[j].x;
+                    a.uy_out[o + gy] = u0[j].y;
+                }
+            }
+        }
+        return;
+    }
+    // tangent: + u(dg) . grad f + u(g) . grad df
+    const float* f1 = a.f1 + (long long)t.b * a.img_stride;
+    ifft_pair<M>(line, base + 2 * pstride, base + 3 * pstride, a.tw, lane);
+#pragma unroll
+    for (int j = 0; j < V; ++j) {
+        const int gy = t.oy0 + lane + kWave * j;
+        if (gy < t.oy1) {
+            const float2 u1 = line[gy - t.p0y];
+            out[gy - e] = u1.x * grad_x(f0, N, gy, gx) + u1.y * grad_y(f0, N, gy, gx) +
+                          u0[j].x * grad_x(f1, N, gy, gx) + u0[j].y * grad_y(f1, N, gy, gx);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// P3 adjoint: per wavefront a pair of output columns (gx, gx+1).
+//   lambda_f(c, r) = (w_x(c-1, r) - w_x(c+1, r))/2 + (w_y(c, r-1) - w_y(c, r+1))/2,
+//   w_d = u_d(g) v0 (+ u_d(dg) v1), zero outside the valid region;
+//   lambda_g = inverse y-FFT of the correlation plane.
+// X2 column index of global column c is c - ox0 + 1.
+// ---------------------------------------------------------------------------
+template <int M>
+__global__ __launch_bounds__(256) void k_p3_adj(EngineArgs a, int nf, int with_u, int planeL)
+{
+    constexpr int V = M / kWave;
+    constexpr int HP = spec_pitch(M);
+    __shared__ float2 lds[4][M];
+    const int lane = lane_id(), w = wave_id();
+    const int slot = blockIdx.y;
+    const Tile t = a.tiles[slot];
+    const int j0 = (blockIdx.x * 4 + w) * 2;
+    const int cnt = t.ox1 - t.ox0;
+    if (j0 >= cnt) return;
+    const bool second = j0 + 1 < cnt;
+    const int N = a.N, e = a.e, W = a.W;
+    const int gx = t.ox0 + j0;
+    const int ncols = cnt + 2;  // stored X2 columns
+    float2* line = lds[w];
+    const long long pstride = (long long)a.S2 * HP;
+    const float2* tbase = a.X2 + (long long)slot * a.np2 * pstride;
+    auto col = [&](int plane, int c) -> const float2* {
+        const int xi2 = c - t.ox0 + 1;
+        return (xi2 >= 0 && xi2 < ncols) ? tbase + plane * pstride + (long long)xi2 * HP : nullptr;
+    };
+    const long long ib = (long long)t.b * a.img_stride;
+
+    if (with_u) {
+        float wx0[V], wx1[V], wx2[V], wx3[V], dy0[V], dy1[V];
+#pragma unroll
+        for (int j = 0; j < V; ++j) wx0[j] = wx1[j] = wx2[j] = wx3[j] = dy0[j] = dy1[j] = 0.f;
+        for (int fi = 0; fi < nf; ++fi) {
+            const float* vv = (fi == 0 ? a.v0 : a.v1) + (long long)t.b * a.w_stride;
+            const int px = 2 * fi, py = 2 * fi + 1;
+            ifft_pair<M>(line, col(px, gx - 1), col(px, gx + 1), a.tw, lane);
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                const int gy = t.oy0 + lane + kWave * j;
+                if (gy < t.oy1) {
+                    const float2 u = line[gy - t.p0y];
+                    wx0[j] += u.x * cot_at(vv, N, e, W, gx - 1, gy);
+                    wx2[j] += u.y * cot_at(vv, N, e, W, gx + 1, gy);
+                }
+            }
+            ifft_pair<M>(line, col(px, gx), col(px, gx + 2), a.tw, lane);
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                const int gy = t.oy0 + lane + kWave * j;
+                if (gy < t.oy1) {
+                    const float2 u = line[gy - t.p0y];
+                    wx1[j] += u.x * cot_at(vv, N, e, W, gx, gy);
+                    wx3[j] += u.y * cot_at(vv, N, e, W, gx + 2, gy);
+                }
+            }
+            ifft_pair<M>(line, col(py, gx), col(py, gx + 1), a.tw, lane);
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                const int gy = t.oy0 + lane + kWave * j;
+                if (gy < t.oy1) {
+                    const int y = gy - t.p0y;
+                    const float2 um = y - 1 >= 0 ? line[y - 1] : make_float2(0.f, 0.f);
+                    const float2 up = y + 1 < M ? line[y + 1] : make_float2(0.f, 0.f);
+                    dy0[j] += um.x * cot_at(vv, N, e, W, gx, gy - 1) -
+                              up.x * cot_at(vv, N, e, W, gx, gy + 1);
+                    dy1[j] += um.y * cot_at(vv, N, e, W, gx + 1, gy - 1) -
+                              up.y * cot_at(vv, N, e, W, gx + 1, gy + 1);
+                }
+            }
+        }
+        float* o0 = a.out0 + ib + (long long)gx * N;
+#pragma unroll
+        for (int j = 0; j < V; ++j) {
+            const int gy = t.oy0 + lane + kWave * j;
+            if (gy < t.oy1) {
+                o0[gy] = 0.5f * (wx0[j] - wx2[j]) + 0.5f * dy0[j];
+                if (second) o0[N + gy] = 0.5f * (wx1[j] - wx3[j]) + 0.5f * dy1[j];
+            }
+        }
+    }
+    // lambda_g
+    ifft_pair<M>(line, col(planeL, gx), col(planeL, gx + 1), a.tw, lane);
+    float* o1 = a.out1 + ib + (long long)gx * N;
+#pragma unroll
+    for (int j = 0; j < V; ++j) {
+        const int gy = t.oy0 + lane + kWave * j;
+        if (gy < t.oy1) {
+            const float2 l = line[gy - t.p0y];
+            o1[gy] = l.x;
+            if (second) o1[N + gy] = l.y;
+        }
+    }
+}
+
+// lambda_f from caller-provided u_x, u_y [W x W] (adjoint/srk_adjoint.cuh:25-59).
+__global__ void k_adjoint_f_from_u(const float* __restrict__ v, const float* __restrict__ ux,
+                                   const float* __restrict__ uy, float* __restrict__ lam,
+                                   int N, int W, int e, long long img_stride, long long w_stride)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    const int b = blockIdx.z;
+    if (r >= N) return;
+    const float* vb = v + b * w_stride;
+    const float* uxb = ux + b * w_stride;
+    const float* uyb = uy + b * w_stride;
+    auto wx = [&](int rr, int cc) -> float {
+        const int orr = rr - e, oc = cc - e;
+        if (orr < 0 || orr >= W || oc < 0 || oc >= W) return 0.f;
+        return uxb[(long long)oc * W + orr] * vb[(long long)oc * W + orr];
+    };
+    auto wy = [&](int rr, int cc) -> float {
+        const int orr = rr - e, oc = cc - e;
+        if (orr < 0 || orr >= W || oc < 0 || oc >= W) return 0.f;
+        return uyb[(long long)oc * W + orr] * vb[(long long)oc * W + orr];
+    };
+    lam[b * img_stride + (long long)c * N + r] =
+        0.5f * (wx(r, c - 1) - wx(r, c + 1)) + 0.5f * (wy(r - 1, c) - wy(r + 1, c));
+}
+
+// GT[d][k_y][k_x] = scale * G_d[k_x*(M/2+1) + k_y]  (32 x 32 LDS-tiled transpose)
+__global__ void k_import_spectra(const float2* __restrict__ Gx, const float2* __restrict__ Gy,
+                                 float2* __restrict__ GT, int M, float scale)
+{
+    __shared__ float2 tile[32][33];
+    const int H = M / 2 + 1;
+    const float2* src = blockIdx.z == 0 ? Gx : Gy;
+    float2* dst = GT + (long long)blockIdx.z * H * M;
+    const int kx0 = blockIdx.x * 32, ky0 = blockIdx.y * 32;
+    for (int i = threadIdx.y; i < 32; i += blockDim.y) {
+        const int kx = kx0 + i, ky = ky0 + threadIdx.x;
+        if (kx < M && ky < H) tile[i][threadIdx.x] = src[(long long)kx * H + ky];
+    }
+    __syncthreads();
+    for (int i = threadIdx.y; i < 32; i += blockDim.y) {
+        const int ky = ky0 + i, kx = kx0 + threadIdx.x;
+        if (kx < M && ky < H) {
+            const float2 g = tile[threadIdx.x][i];
+            dst[(long long)ky * M + kx] = make_float2(g.x * scale, g.y * scale);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+template <int M>
+struct P2Cfg;
+template <>
+struct P2Cfg<256> { static constexpr int RY = 16, NW = 8; };
+template <>
+struct P2Cfg<512> { static constexpr int RY = 16, NW = 8; };
+template <>
+struct P2Cfg<1024> { static constexpr int RY = 8, NW = 8; };
+template <>
+struct P2Cfg<2048> { static constexpr int RY = 4, NW = 4; };
+
+template <int M>
+constexpr size_t p2_lds()
+{
+    return 2ull * P2Cfg<M>::RY * (M + 1) * sizeof(float2);
+}
+
+template <int M>
+hipError_t prepare_kernels()
+{
+    static bool done = false;
+    static hipError_t st = hipSuccess;
+    if (done) return st;
+    constexpr int RY = P2Cfg<M>::RY, NW = P2Cfg<M>::NW;
+    st = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_p2_conv<M, RY, NW>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)p2_lds<M>());
+    if (st == hipSuccess)
+        st = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_p2_corr<M, RY, NW>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)p2_lds<M>());
+    done = true;
+    return st;
+}
+
+template <int M>
+int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s)
+{
+    constexpr int H = M / 2 + 1;
+    constexpr int RY = P2Cfg<M>::RY, NW = P2Cfg<M>::NW;
+    hipError_t st = prepare_kernels<M>();
+    if (st != hipSuccess) return (int)st;
+    const size_t lds2 = p2_lds<M>();
+    const int cols_out = a0.S2 - 2 * a0.halo2;  // max output columns per tile
+    for (int t0 = 0; t0 < ntiles; t0 += chunk) {
+        const int c = std::min(chunk, ntiles - t0);
+        EngineArgs a = a0;
+        a.tiles = a0.tiles + t0;
+        const dim3 g1(M / 8, c), g1a(M / 4, c), g2((H + RY - 1) / RY, c);
+        const dim3 g3f((cols_out + 3) / 4, c), g3a(((cols_out + 1) / 2 + 3) / 4, c);
+        const dim3 b256(256), b2(NW * kWave);
+        switch (op) {
+        case Op::Forward:
+            k_p1_cols<M><<<g1, b256, 0, s>>>(a, 1);
+            k_p2_conv<M, RY, NW><<<g2, b2, lds2, s>>>(a, 1);
+            k_p3_fwd<M><<<g3f, b256, 0, s>>>(a, 1);
+            break;
+        case Op::Tangent:
+            k_p1_cols<M><<<g1, b256, 0, s>>>(a, 2);
+            k_p2_conv<M, RY, NW><<<g2, b2, lds2, s>>>(a, 2);
+            k_p3_fwd<M><<<g3f, b256, 0, s>>>(a, 2);
+            break;
+        case Op::Adjoint:
+            k_p1_cols<M><<<g1, b256, 0, s>>>(a, 1);
+            k_p1_adj_src<M><<<g1a, b256, 0, s>>>(a, 1, 1);
+            k_p2_conv<M, RY, NW><<<g2, b2, lds2, s>>>(a, 1);
+            k_p2_corr<M, RY, NW><<<g2, b2, lds2, s>>>(a, 1, 2);
+            k_p3_adj<M><<<g3a, b256, 0, s>>>(a, 1, 1, 2);
+            break;
+        case Op::AdjointG:
+            k_p1_adj_src<M><<<g1a, b256, 0, s>>>(a, 1, 0);
+            k_p2_corr<M, RY, NW><<<g2, b2, lds2, s>>>(a, 0, 0);
+            k_p3_adj<M><<<g3a, b256, 0, s>>>(a, 1, 0, 0);
+            break;
+        case Op::TangentAdjoint:
+            k_p1_cols<M><<<g1, b256, 0, s>>>(a, 2);
+            k_p1_adj_src<M><<<g1a, b256, 0, s>>>(a, 2, 2);
+            k_p2_conv<M, RY, NW><<<g2, b2, lds2, s>>>(a, 2);
+            k_p2_corr<M, RY, NW><<<g2, b2, lds2, s>>>(a, 2, 4);
+            k_p3_adj<M><<<g3a, b256, 0, s>>>(a, 2, 1, 4);
+            break;
+        }
+        st = hipGetLastError();
+        if (st != hipSuccess) return (int)st;
+    }
+    return 0;
+}
+
+}  // namespace
+
+int engine_run(Op op, int M, const EngineArgs& a, int ntiles, int chunk, hipStream_t s)
+{
+    if (chunk < 1 || ntiles < 0) return (int)hipErrorInvalidValue;
+    if (ntiles == 0) return 0;
+    switch (M) {
+    case 256: return run_impl<256>(op, a, ntiles, chunk, s);
+    case 512: return run_impl<512>(op, a, ntiles, chunk, s);
+    case 1024: return run_impl<1024>(op, a, ntiles, chunk, s);
+    case 2048: return run_impl<2048>(op, a, ntiles, chunk, s);
+    default: return (int)hipErrorInvalidValue;
+    }
+}
+
+size_t engine_p2_lds_bytes(int M)
+{
+    switch (M) {
+    case 256: return p2_lds<256>();
+    case 512: return p2_lds<512>();
+    case 1024: return p2_lds<1024>();
+    case 2048: return p2_lds<2048>();
+    default: return 0;
+    }
+}
+
+int engine_rows_per_block(int M)
+{
+    switch (M) {
+    case 256: return P2Cfg<256>::RY;
+    case 512: return P2Cfg<512>::RY;
+    case 1024: return P2Cfg<1024>::RY;
+    case 2048: return P2Cfg<2048>::RY;
+    default: return 0;
+    }
+}
+
+int engine_import_spectra(int M, const float2* Gx_ref, const float2* Gy_ref, float scale,
+                          float2* GT, hipStream_t s)
+{
+    const int H = M / 2 + 1;
+    dim3 grid((M + 31) / 32, (H + 31) / 32, 2), block(32, 8);
+    k_import_spectra<<<grid, block, 0, s>>>(Gx_ref, Gy_ref, GT, M, scale);
+    return (int)hipGetLastError();
+}
+
+int engine_adjoint_f_from_u(const float* v, const float* ux, const float* uy, float* lam_f,
+                            int N, int W, int e, int B, long long img_stride,
+                            long long w_stride, hipStream_t s)
+{
+    dim3 block(256), grid((N + 255) / 256, N, B);
+    k_adjoint_f_from_u<<<grid, block, 0, s>>>(v, ux, uy, lam_f, N, W, e, img_stride, w_stride);
+    return (int)hipGetLastError();
+}
+
+}  // namespace srk

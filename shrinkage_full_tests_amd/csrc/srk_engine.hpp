@@ -1,0 +1,88 @@
+// srk_engine.hpp -- host/device interface of the overlap-and-save tile engine.
+//
+// The engine evaluates the shrinkage operator family on M x M tiles in three
+// passes, with exchange buffers between them sized for a chunk of tiles:
+//
+//   P1  (columns)  real FFT along y of every tile column (contiguous loads),
+//                  written column-spectrum-major: X1[tile][plane][x][k_y];
+//   P2  (rows)     per k_y band: FFT along x, product with the kernel
+//                  spectra, inverse FFT along x, crop to the needed columns:
+//                  X2[tile][plane][x][k_y];
+//   P3  (columns)  inverse FFT along y, fused with the gradient stencil, the
+//                  combine / adjoint stencil and the valid-region crop; writes
+//                  the caller's output directly.
+//
+// Spectra are held row-major by k_y (GT[d][k_y][k_x]) and pre-scaled by
+// dx^2 / M^2, so P2 reads them coalesced along k_x.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace srk {
+
+// Geometry of one tile, in global image coordinates (pointer free, so tile
+// tables are built once per workspace).
+struct Tile {
+    int b;             // image index within the batch
+    int p0x, p0y;      // window origin (column, row); may be negative
+    int ox0, ox1;      // output columns [ox0, ox1)
+    int oy0, oy1;      // output rows    [oy0, oy1)
+    int pad;
+};
+
+// Arguments shared by every pass (passed by value).
+struct EngineArgs {
+    const float* g0;   // convolved field (g)           [B][N][N]
+    const float* g1;   // second convolved field (dg)   [B][N][N] or null
+    const float* f0;   // differentiated field (f)      [B][N][N]
+    const float* f1;   // second differentiated field (df) or null
+    const float* v0;   // cotangent (v, or d lambda_h)  [B][W][W]
+    const float* v1;   // second cotangent (lambda_h)   [B][W][W] or null
+    float* out0;       // h / dh [B][W][W], or lambda_f [B][N][N]
+    float* out1;       // lambda_g [B][N][N]
+    float* ux_out;     // optional u_x = s conv(g, Gx) on the valid region [B][W][W]
+    float* uy_out;     // optional u_y
+    long long img_stride;  // elements between images of N x N
+    long long w_stride;    // elements between images of W x W
+    int N, e, W;
+    const float2* GT;      // [2][M/2+1][M] scaled spectra (Gx then Gy)
+    const float2* tw;      // twiddles exp(-2 pi i k / M), k < M
+    const Tile* tiles;     // tiles of this chunk
+    float2* X1;            // [chunk][np1][M][HP]
+    float2* X2;            // [chunk][np2][S2][HP]
+    int np1, np2;          // planes per tile in X1 / X2
+    int S2;                // columns per plane in X2
+    int halo2;             // extra stored columns on each side in X2 (0 fwd, 1 adj)
+};
+
+// Padded pitch (complex elements) of one column spectrum in X1/X2.
+constexpr int spec_pitch(int M) { return M / 2 + 16; }
+
+enum class Op : int {
+    Forward = 0,         // h = u(g) . grad f
+    Tangent = 1,         // dh = u(dg) . grad f + u(g) . grad df
+    Adjoint = 2,         // (lambda_f, lambda_g), u recomputed
+    AdjointG = 3,        // lambda_g only (lambda_f from caller-provided u)
+    TangentAdjoint = 4,  // (d lambda_f, d lambda_g)
+};
+
+// Launch the passes of `op` for tiles [0, ntiles) of the tile table
+// args.tiles, chunk tiles at a time.  Returns a hipError_t.
+int engine_run(Op op, int M, const EngineArgs& args, int ntiles, int chunk, hipStream_t stream);
+
+// Transpose + scale reference-layout spectra (Ghat[k_x*(M/2+1)+k_y]) into GT.
+int engine_import_spectra(int M, const float2* Gx_ref, const float2* Gy_ref, float scale,
+                          float2* GT, hipStream_t stream);
+
+// lambda_f from caller-provided u (adjoint of the gradient stencil + combine).
+int engine_adjoint_f_from_u(const float* v, const float* ux, const float* uy, float* lam_f,
+                            int N, int W, int e, int B, long long img_stride,
+                            long long w_stride, hipStream_t stream);
+
+// Bytes of LDS the P2 kernels use for a given M (for capability checks).
+size_t engine_p2_lds_bytes(int M);
+int engine_rows_per_block(int M);
+
+}  // namespace srk

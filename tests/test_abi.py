@@ -1,0 +1,91 @@
+"""The C-ABI library loads, exports every symbol include/srk.h declares, and
+rejects bad arguments before touching the GPU (CPU only)."""
+import ctypes
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+from shrinkage_full_tests_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "srk.h")
+
+
+def _declared():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(\w+)\s*\(", src, flags=re.M)))
+
+
+def test_header_declares_the_reference_entry_points():
+    names = _declared()
+    for n in ("srkTrainingForward", "adjointSRK", "shrinkageTangentOaS",
+              "shrinkageTangentAdjointOaS", "srkForwardOaS", "srkPrecomputeKernelFreq"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = _declared()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(_lib.SIGNATURES), "ctypes signatures out of sync with srk.h"
+
+
+def test_error_strings():
+    L = _lib.lib()
+    for code in range(0, -10, -1):
+        s = L.srkGetErrorString(code).decode()
+        assert s and s != "unknown status"
+    assert L.srkGetErrorString(-99).decode() == "unknown status"
+
+
+def _create(N, W, e, M, B=1):
+    h = ctypes.c_void_p()
+    st = _lib.lib().srkWorkspaceCreate(N, W, e, M, B, 0, ctypes.byref(h))
+    return st, h
+
+
+def test_workspace_argument_errors():
+    assert _create(256, 200, 15, 256)[0] == -2        # W != N - 2e
+    assert _create(256, 256, 0, 256)[0] == -2         # erosion < 1
+    assert _create(30, -2, 16, 256)[0] == -2          # W <= 0
+    assert _create(256, 226, 15, 300)[0] == -5        # M not supported
+    assert _create(1024, 904, 60, 128)[0] == -5       # M not supported
+    assert _create(1024, 770, 127, 256)[0] == -5      # M <= 2e + 2
+    assert _create(256, 226, 15, 256, B=0)[0] == -7   # B_max < 1
+    assert _lib.lib().srkWorkspaceCreate(256, 226, 15, 256, 1, 0, None) == -3
+
+
+def test_entry_points_reject_null_workspace():
+    L = _lib.lib()
+    assert L.srkTrainingForward(1, 1, 8, 8, 256, 226, 15, 1.0, 1, None, 1, None) == -3
+    assert L.srkForwardBatched(1, 1, 8, 8, 256, 226, 15, 1.0, 1, None, 1, None) == -3
+    assert L.srkAdjointBatched(1, 1, 1, 8, 8, 256, 226, 15, 1.0, 1, None, 1, 1, None) == -3
+    assert L.shrinkageTangentOaS(1, 1, 1, 1, 8, 8, 256, 226, 15, 1.0, None, 1, None) == -3
+
+
+def test_kernel_spectra_argument_errors():
+    L = _lib.lib()
+    assert L.srkKernelSpectraSRSRK(30, 5.0, 1.0, 3.0, 256, None, None, None, None, None) == -9
+    assert L.srkKernelSpectraSRSRK(31, -1.0, 1.0, 3.0, 256, None, None, None, None, None) == -9
+    assert L.srkKernelSpectraSRSRK(31, 5.0, 1.0, 3.0, 300, None, None, None, None, None) == -5
+    assert L.srkPrecomputeKernelFreq(None, 31, 31, 256, None, None) == -3
+
+
+@pytest.mark.parametrize("kw,sigma", [(31, 5.0), (61, 10.0), (121, 20.0)])
+def test_product_kernel_generation_matches_scipy(kw, sigma):
+    """The library's own FP64 K1 / Green's tensor (host code) against scipy goldens."""
+    g = json.load(open(os.path.join(ROOT, "tests", "golden", "k1_scipy.json")))
+    hx = np.zeros((kw, kw), np.float32)
+    hy = np.zeros((kw, kw), np.float32)
+    st = _lib.lib().srkKernelSpectraSRSRK(kw, sigma, 1.0, 3.0, 256, None, None,
+                                          hx.ctypes.data, hy.ctypes.data, None)
+    assert st == 0
+    for i, j, kx, ky in g["srsrk_samples"][f"{kw}_{sigma}"]:
+        assert abs(hx[i, j] - kx) <= 1e-6 * abs(kx) + 1e-30
+        assert abs(hy[i, j] - ky) <= 1e-6 * abs(ky) + 1e-30
