@@ -1,0 +1,242 @@
+"""Benchmark: SRK shrinkage forward + adjoint throughput on one MI355X.
+
+Workload (one step): srkTrainingForward over B=5000 images of N=512 (E = I,
+Gaussian random fields), KW=121 (erosion 60, W=392), FP32, followed by the
+batched adjoint (lambda_f, lambda_g) for a random cotangent with u recomputed.
+Metric: input Mpix/s = B*N^2 / (t_fwd + t_adj).  A second point, N=16384
+single image (f != g), is reported under "secondary".
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def alg_bytes(N, W, B, e_eq_i):
+    """Compulsory fp32 bytes of one forward + adjoint (u recomputed):
+    f != g: read f, g (8N^2), write h (4W^2); read f, g, v (8N^2 + 4W^2), write
+    lambda_f, lambda_g (8N^2) -> 24N^2 + 8W^2.  E = I: 16N^2 + 8W^2."""
+    per = (16 if e_eq_i else 24) * N * N + 8 * W * W
+    return per * B
+
+
+def cpu_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except Exception:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(N, kw, sigma, e, budget_s=12.0):
+    """Timed FP64 FFT-form forward + adjoint of the oracle on the host cores."""
+    import numpy as np
+    import oracle
+    from shrinkage_full_tests_amd import patterns
+    threads = min(16, cpu_cores())
+    oracle.set_threads(threads)
+    Kx, Ky = oracle.srsrk_kernel(kw, 1.0, sigma)
+    W = N - 2 * e
+    f = patterns.grf(N, 1000, B=1).astype(np.float64)
+    v = np.random.default_rng(5).standard_normal((1, W, W))
+    t0 = time.perf_counter()
+    oracle.fwd_adj_fft_batch(f, f, v, e, Kx, Ky)
+    t1 = time.perf_counter() - t0
+    n = max(1, min(64, int(budget_s / max(t1, 1e-3))))
+    fb = np.repeat(f, n, axis=0)
+    vb = np.repeat(v, n, axis=0)
+    t0 = time.perf_counter()
+    oracle.fwd_adj_fft_batch(fb, fb, vb, e, Kx, Ky)
+    dt = time.perf_counter() - t0
+    return {"value": round(n * N * N / dt / 1e6, 3), "unit": "Mpix/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{n} image(s) N={N} KW={kw} E=I, FP64 FFT-form fwd+adj (oracle), "
+                      f"{dt:.1f} s on {platform.processor() or platform.machine()}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--b", type=int, default=5000)
+    ap.add_argument("--kw", type=int, default=121)
+    ap.add_argument("--m", type=int, default=0, help="tile/FFT size (default N)")
+    ap.add_argument("--chunk", type=int, default=0, help="tiles per chunk (0 = auto)")
+    ap.add_argument("--scratch-mb", type=int, default=0)
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--large-n", type=int, default=16384)
+    ap.add_argument("--large-m", type=int, default=1024)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the shrinkage operator has no multi-GPU path "
+              "in this build (batch sharding across GPUs is a design note, not code). "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import shrinkage_full_tests_amd as srk
+    from shrinkage_full_tests_amd import patterns
+
+    torch.cuda.set_device(0)
+    srk.lib()  # fails loudly if the HIP library is missing
+    N, B, kw = args.n, args.b, args.kw
+    e = (kw - 1) // 2
+    W = N - 2 * e
+    sigma = e / 3.0
+    M = args.m or N
+    dx = 1.0
+
+    gx, gy = srk.kernel_spectra_srsrk(kw, sigma, M)
+    E = patterns.grf_torch(B, N, seed=1000)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(7)
+    v = torch.randn((B, W, W), generator=gen, device="cuda")
+    R = torch.empty((B, W, W), device="cuda")
+    lf = torch.empty((B, N, N), device="cuda")
+    lg = torch.empty((B, N, N), device="cuda")
+    ws = srk.Workspace(N, W, e, M, B_max=B, scratch_mb=args.scratch_mb)
+    if args.chunk:
+        ws.set_chunk(args.chunk)
+
+    def fwd():
+        if M == N:
+            srk.srkTrainingForward(E, E, gx, gy, N, W, e, dx, B, ws, R)
+        else:
+            srk.forward_batched(E, E, gx, gy, e, dx, ws, R)
+
+    def adj():
+        srk.adjoint_batched(v, E, E, gx, gy, e, dx, ws, lf, lg)
+
+    def step():
+        fwd()
+        adj()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    graph = None
+    if not args.no_graph:
+        graph = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(graph, stream=s):
+                step()
+        torch.cuda.current_stream().wait_stream(s)
+        graph.replay()
+        torch.cuda.synchronize()
+
+    run = graph.replay if graph is not None else step
+    stream = torch.cuda.current_stream()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for _ in range(args.steps):
+        run()
+    stop.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / args.steps
+
+    # forward / adjoint split (eager, same stream)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    reps = max(1, min(3, args.steps))
+    t_f = t_a = 0.0
+    for _ in range(reps):
+        ev[0].record(stream)
+        fwd()
+        ev[1].record(stream)
+        adj()
+        ev[2].record(stream)
+        torch.cuda.synchronize()
+        t_f += ev[0].elapsed_time(ev[1]) / reps
+        t_a += ev[1].elapsed_time(ev[2]) / reps
+
+    value = B * N * N / (ms / 1e3) / 1e6
+    nbytes = alg_bytes(N, W, B, True)
+    achieved = nbytes / (ms / 1e3) / 1e9
+    info = ws.info()
+
+    secondary = None
+    if not args.no_large:
+        del lf, lg, R, v, E, ws
+        torch.cuda.empty_cache()
+        NL, ML = args.large_n, args.large_m
+        WL = NL - 2 * e
+        gxl, gyl = srk.kernel_spectra_srsrk(kw, sigma, ML)
+        fL = patterns.grf_torch(1, NL, seed=3000, sigma=8.0)
+        gL = patterns.grf_torch(1, NL, seed=4000, sigma=8.0)
+        vL = torch.randn((1, WL, WL), generator=gen, device="cuda")
+        hL = torch.empty((1, WL, WL), device="cuda")
+        lfL = torch.empty((1, NL, NL), device="cuda")
+        lgL = torch.empty((1, NL, NL), device="cuda")
+        wsL = srk.Workspace(NL, WL, e, ML, B_max=1)
+
+        def stepL():
+            srk.forward_batched(gL, fL, gxl, gyl, e, dx, wsL, hL)
+            srk.adjoint_batched(vL, gL, fL, gxl, gyl, e, dx, wsL, lfL, lgL)
+
+        stepL()
+        torch.cuda.synchronize()
+        start.record(stream)
+        for _ in range(3):
+            stepL()
+        stop.record(stream)
+        torch.cuda.synchronize()
+        msL = start.elapsed_time(stop) / 3
+        bL = alg_bytes(NL, WL, 1, False)
+        secondary = {"workload": f"fwd+adj N={NL} B=1 KW={kw} f!=g, M={ML}",
+                     "value": round(NL * NL / (msL / 1e3) / 1e6, 1), "unit": "Mpix/s",
+                     "ms_per_step": round(msL, 3),
+                     "hbm_frac": round(bL / (msL / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    cpu = None if args.no_cpu else cpu_baseline(N, kw, sigma, e)
+
+    out = {
+        "metric": f"fwd+adjoint Mpix/s, N={N} B={B} KW={kw}",
+        "value": round(value, 1),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Gaussian random fields, random cotangent)",
+        "config": {"workload": f"srkTrainingForward + srkAdjointBatched (u recomputed), "
+                               f"N={N} B={B} KW={kw} erosion={e} W={W} E=I, FP32",
+                   "N": N, "B": B, "KW": kw, "erosion": e, "W": W, "M": M,
+                   "tiles_per_chunk_fwd": info["chunk_fwd"], "tiles_per_chunk_adj": info["chunk_adj"],
+                   "hipgraph": graph is not None},
+        "ms_fwd": round(t_f, 3),
+        "ms_adj": round(t_a, 3),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None,
+                     "scope": "one fwd+adj step (all passes), algorithmic bytes "
+                              f"16N^2+8W^2 per image = {nbytes / 1e9:.2f} GB"},
+        "secondary": secondary,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -54,6 +54,8 @@ def lib():
         L.orc_tangent.argtypes = [_D, _D, _D, _D, i, i, _D, _D, i, d, _D]
         L.orc_tangent_adjoint.argtypes = [_D, _D, _D, _D, _D, _D, i, i, _D, _D, i, d,
                                           _D, _D]
+        L.orc_forward_window.argtypes = [_D, _D, i, i, _D, _D, i, d, i, i, i, i, _D]
+        L.orc_adjoint_window.argtypes = [_D, _D, _D, i, i, _D, _D, i, d, i, i, i, i, _D, _D]
         L.orc_fwd_adj_fft_batch.argtypes = [_D, _D, _D, i, i, i, _D, _D, i, d, _D, _D, _D]
         _lib = L
     return _lib
@@ -160,3 +162,21 @@ def fwd_adj_fft_batch(f, g, v, e, Kx, Ky, s=1.0):
     lib().orc_fwd_adj_fft_batch(_p(f), _p(g), _p(v), B, N, int(e), _p(Kx), _p(Ky), Kx.shape[0],
                                 float(s), _p(h), _p(lf), _p(lg))
     return h, lf, lg
+
+
+def forward_window(f, g, e, Kx, Ky, ox0, oy0, wx, wy, s=1.0):
+    """h on output window [ox0, ox0+wx) x [oy0, oy0+wy) (valid-region coordinates)."""
+    f = _f64(f); g = _f64(g); Kx = _f64(Kx); Ky = _f64(Ky)
+    out = np.zeros((wx, wy))
+    lib().orc_forward_window(_p(f), _p(g), f.shape[0], int(e), _p(Kx), _p(Ky), Kx.shape[0],
+                             float(s), int(ox0), int(oy0), int(wx), int(wy), _p(out))
+    return out
+
+
+def adjoint_window(v, f, g, e, Kx, Ky, qx0, qy0, wx, wy, s=1.0):
+    """(lambda_f, lambda_g) on image window [qx0, qx0+wx) x [qy0, qy0+wy)."""
+    v = _f64(v); f = _f64(f); g = _f64(g); Kx = _f64(Kx); Ky = _f64(Ky)
+    a = np.zeros((wx, wy)); b = np.zeros((wx, wy))
+    lib().orc_adjoint_window(_p(v), _p(f), _p(g), f.shape[0], int(e), _p(Kx), _p(Ky), Kx.shape[0],
+                             float(s), int(qx0), int(qy0), int(wx), int(wy), _p(a), _p(b))
+    return a, b

@@ -394,6 +394,106 @@ void orc_tangent_adjoint(const double* f, const double* g, const double* df,
     free(tg);
 }
 
+/*
+ * Windowed direct forms for large images (restating the same sums as
+ * orc_forward / orc_adjoint, evaluated only on a window):
+ *   orc_forward_window: h(o) for output pixels o in [ox0, ox0+wx) x [oy0, oy0+wy)
+ *     (output = valid-region coordinates), h_out[(ox-ox0)*wy + (oy-oy0)].
+ *   orc_adjoint_window: lambda_f, lambda_g at image pixels q in
+ *     [qx0, qx0+wx) x [qy0, qy0+wy), out[(qx-qx0)*wy + (qy-qy0)].
+ * Pinned by: equality with orc_forward / orc_adjoint on whole small images.
+ */
+static double u_at(const double* g, int N, const double* K, int kw, int gx, int gy)
+{
+    const int ck = kw / 2;
+    double acc = 0.0;
+    for (int kx = -ck; kx <= ck; ++kx) {
+        int x = gx - kx;
+        if (x < 0 || x >= N) continue;
+        for (int ky = -ck; ky <= ck; ++ky) {
+            int y = gy - ky;
+            if (y < 0 || y >= N) continue;
+            acc += g[(size_t)x * N + y] * K[(size_t)(kx + ck) * kw + ky + ck];
+        }
+    }
+    return acc;
+}
+
+typedef struct {
+    const double *f, *g, *v, *Kx, *Ky;
+    int N, e, kw, x0, y0, wx, wy;
+    double s;
+    double *o0, *o1;
+} win_ctx;
+
+static void fwd_win_body(void* p, int64_t a, int64_t b)
+{
+    win_ctx* C = (win_ctx*)p;
+    for (int64_t i = a; i < b; ++i) {
+        int ox = C->x0 + (int)(i / C->wy), oy = C->y0 + (int)(i % C->wy);
+        int gx = ox + C->e, gy = oy + C->e;
+        double ux = C->s * u_at(C->g, C->N, C->Kx, C->kw, gx, gy);
+        double uy = C->s * u_at(C->g, C->N, C->Ky, C->kw, gx, gy);
+        C->o0[i] = ux * fx_at(C->f, C->N, gy, gx) + uy * fy_at(C->f, C->N, gy, gx);
+    }
+}
+
+void orc_forward_window(const double* f, const double* g, int N, int e, const double* Kx,
+                        const double* Ky, int kw, double s, int ox0, int oy0, int wx, int wy,
+                        double* h_out)
+{
+    win_ctx C = {f, g, NULL, Kx, Ky, N, e, kw, ox0, oy0, wx, wy, s, h_out, NULL};
+    parallel_for((int64_t)wx * wy, fwd_win_body, &C);
+}
+
+static double cot(const double* v, int N, int e, int c, int r)
+{
+    const int W = N - 2 * e;
+    int oc = c - e, orr = r - e;
+    if (oc < 0 || oc >= W || orr < 0 || orr >= W) return 0.0;
+    return v[(size_t)oc * W + orr];
+}
+
+static void adj_win_body(void* p, int64_t a, int64_t b)
+{
+    win_ctx* C = (win_ctx*)p;
+    const int N = C->N, e = C->e, kw = C->kw, ck = kw / 2;
+    for (int64_t i = a; i < b; ++i) {
+        int qx = C->x0 + (int)(i / C->wy), qy = C->y0 + (int)(i % C->wy);
+        /* lambda_f: stencil transpose of w_d = u_d v */
+        double wxm = cot(C->v, N, e, qx - 1, qy), wxp = cot(C->v, N, e, qx + 1, qy);
+        double wym = cot(C->v, N, e, qx, qy - 1), wyp = cot(C->v, N, e, qx, qy + 1);
+        double lf = 0.0;
+        if (wxm != 0.0) lf += 0.5 * C->s * u_at(C->g, N, C->Kx, kw, qx - 1, qy) * wxm;
+        if (wxp != 0.0) lf -= 0.5 * C->s * u_at(C->g, N, C->Kx, kw, qx + 1, qy) * wxp;
+        if (wym != 0.0) lf += 0.5 * C->s * u_at(C->g, N, C->Ky, kw, qx, qy - 1) * wym;
+        if (wyp != 0.0) lf -= 0.5 * C->s * u_at(C->g, N, C->Ky, kw, qx, qy + 1) * wyp;
+        C->o0[i] = lf;
+        /* lambda_g: correlation of a_d = v f_d with G_d */
+        double lg = 0.0;
+        for (int kx = -ck; kx <= ck; ++kx) {
+            int pc = qx + kx;
+            if (pc < e || pc >= N - e) continue;
+            for (int ky = -ck; ky <= ck; ++ky) {
+                int pr = qy + ky;
+                if (pr < e || pr >= N - e) continue;
+                double vv = cot(C->v, N, e, pc, pr);
+                lg += vv * (fx_at(C->f, N, pr, pc) * C->Kx[(size_t)(kx + ck) * kw + ky + ck] +
+                            fy_at(C->f, N, pr, pc) * C->Ky[(size_t)(kx + ck) * kw + ky + ck]);
+            }
+        }
+        C->o1[i] = C->s * lg;
+    }
+}
+
+void orc_adjoint_window(const double* v, const double* f, const double* g, int N, int e,
+                        const double* Kx, const double* Ky, int kw, double s, int qx0, int qy0,
+                        int wx, int wy, double* lam_f, double* lam_g)
+{
+    win_ctx C = {f, g, v, Kx, Ky, N, e, kw, qx0, qy0, wx, wy, s, lam_f, lam_g};
+    parallel_for((int64_t)wx * wy, adj_win_body, &C);
+}
+
 /* ------------------------------------------------------------------------ */
 /* FP64 FFT path (the timed CPU baseline).  Radix-2 iterative complex FFT,    */
 /* 2-D by lines, multithreaded over lines.  Whole-image circular convolution  */

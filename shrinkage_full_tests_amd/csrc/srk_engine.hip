@@ -377,70 +377,84 @@ __global__ __launch_bounds__(256) void k_p3_adj(EngineArgs a, int nf, int with_u
     const bool second = j0 + 1 < cnt;
     const int N = a.N, e = a.e, W = a.W;
     const int gx = t.ox0 + j0;
-    const int ncols = cnt + 2;  // stored X2 columns
     float2* line = lds[w];
     const long long pstride = (long long)a.S2 * HP;
     const float2* tbase = a.X2 + (long long)slot * a.np2 * pstride;
-    auto col = [&](int plane, int c) -> const float2* {
-        const int xi2 = c - t.ox0 + 1;
-        return (xi2 >= 0 && xi2 < ncols) ? tbase + plane * pstride + (long long)xi2 * HP : nullptr;
+    // X2 column of global column c is c - ox0 + 1; cnt + 2 columns are stored.
+    const int xi0 = j0 + 1;
+    const int ncols = cnt + 2;
+    auto col = [&](int plane, int dc) -> const float2* {
+        const int xi2 = xi0 + dc;
+        return xi2 < ncols ? tbase + plane * pstride + (long long)xi2 * HP : nullptr;
     };
     const long long ib = (long long)t.b * a.img_stride;
 
     if (with_u) {
-        float wx0[V], wx1[V], wx2[V], wx3[V], dy0[V], dy1[V];
+        // lambda_f at columns gx (acc0) and gx+1 (acc1) for tile-local rows
+        // y = lane + 64 j, accumulated transform by transform.
+        float acc0[V], acc1[V];
 #pragma unroll
-        for (int j = 0; j < V; ++j) wx0[j] = wx1[j] = wx2[j] = wx3[j] = dy0[j] = dy1[j] = 0.f;
+        for (int j = 0; j < V; ++j) acc0[j] = acc1[j] = 0.f;
         for (int fi = 0; fi < nf; ++fi) {
             const float* vv = (fi == 0 ? a.v0 : a.v1) + (long long)t.b * a.w_stride;
+            // cotangent columns gx-1 .. gx+2 at this lane's rows (zero off the valid region)
+            float cv[4][V];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int cc = gx - 1 + c;
+                const bool colv = cc >= e && cc < N - e;
+                const float* vcol = vv + (long long)(cc - e) * W - e;
+#pragma unroll
+                for (int j = 0; j < V; ++j) {
+                    const int gy = t.p0y + lane + kWave * j;
+                    cv[c][j] = (colv && gy >= e && gy < N - e) ? vcol[gy] : 0.f;
+                }
+            }
             const int px = 2 * fi, py = 2 * fi + 1;
-            ifft_pair<M>(line, col(px, gx - 1), col(px, gx + 1), a.tw, lane);
+            // u_x at gx-1, gx+1 -> x-stencil transpose at gx
+            ifft_pair<M>(line, col(px, -1), col(px, 1), a.tw, lane);
 #pragma unroll
             for (int j = 0; j < V; ++j) {
-                const int gy = t.oy0 + lane + kWave * j;
-                if (gy < t.oy1) {
-                    const float2 u = line[gy - t.p0y];
-                    wx0[j] += u.x * cot_at(vv, N, e, W, gx - 1, gy);
-                    wx2[j] += u.y * cot_at(vv, N, e, W, gx + 1, gy);
-                }
+                const float2 u = line[lane + kWave * j];
+                acc0[j] += 0.5f * (u.x * cv[0][j] - u.y * cv[2][j]);
             }
-            ifft_pair<M>(line, col(px, gx), col(px, gx + 2), a.tw, lane);
+            // u_x at gx, gx+2 -> x-stencil transpose at gx+1
+            ifft_pair<M>(line, col(px, 0), col(px, 2), a.tw, lane);
 #pragma unroll
             for (int j = 0; j < V; ++j) {
-                const int gy = t.oy0 + lane + kWave * j;
-                if (gy < t.oy1) {
-                    const float2 u = line[gy - t.p0y];
-                    wx1[j] += u.x * cot_at(vv, N, e, W, gx, gy);
-                    wx3[j] += u.y * cot_at(vv, N, e, W, gx + 2, gy);
-                }
+                const float2 u = line[lane + kWave * j];
+                acc1[j] += 0.5f * (u.x * cv[1][j] - u.y * cv[3][j]);
             }
-            ifft_pair<M>(line, col(py, gx), col(py, gx + 1), a.tw, lane);
+            // u_y at gx, gx+1: w_y = u_y v formed in place, then the y-stencil
+            // transpose reads the neighbouring rows (same wavefront, in order).
+            ifft_pair<M>(line, col(py, 0), col(py, 1), a.tw, lane);
 #pragma unroll
             for (int j = 0; j < V; ++j) {
-                const int gy = t.oy0 + lane + kWave * j;
-                if (gy < t.oy1) {
-                    const int y = gy - t.p0y;
-                    const float2 um = y - 1 >= 0 ? line[y - 1] : make_float2(0.f, 0.f);
-                    const float2 up = y + 1 < M ? line[y + 1] : make_float2(0.f, 0.f);
-                    dy0[j] += um.x * cot_at(vv, N, e, W, gx, gy - 1) -
-                              up.x * cot_at(vv, N, e, W, gx, gy + 1);
-                    dy1[j] += um.y * cot_at(vv, N, e, W, gx + 1, gy - 1) -
-                              up.y * cot_at(vv, N, e, W, gx + 1, gy + 1);
-                }
+                const int y = lane + kWave * j;
+                const float2 u = line[y];
+                line[y] = make_float2(u.x * cv[1][j], u.y * cv[2][j]);
+            }
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                const int y = lane + kWave * j;
+                const float2 wm = y - 1 >= 0 ? line[y - 1] : make_float2(0.f, 0.f);
+                const float2 wp = y + 1 < M ? line[y + 1] : make_float2(0.f, 0.f);
+                acc0[j] += 0.5f * (wm.x - wp.x);
+                acc1[j] += 0.5f * (wm.y - wp.y);
             }
         }
         float* o0 = a.out0 + ib + (long long)gx * N;
 #pragma unroll
         for (int j = 0; j < V; ++j) {
-            const int gy = t.oy0 + lane + kWave * j;
-            if (gy < t.oy1) {
-                o0[gy] = 0.5f * (wx0[j] - wx2[j]) + 0.5f * dy0[j];
-                if (second) o0[N + gy] = 0.5f * (wx1[j] - wx3[j]) + 0.5f * dy1[j];
+            const int gy = t.p0y + lane + kWave * j;
+            if (gy >= t.oy0 && gy < t.oy1) {
+                o0[gy] = acc0[j];
+                if (second) o0[N + gy] = acc1[j];
             }
         }
     }
     // lambda_g
-    ifft_pair<M>(line, col(planeL, gx), col(planeL, gx + 1), a.tw, lane);
+    ifft_pair<M>(line, col(planeL, 0), col(planeL, 1), a.tw, lane);
     float* o1 = a.out1 + ib + (long long)gx * N;
 #pragma unroll
     for (int j = 0; j < V; ++j) {
@@ -508,9 +522,9 @@ __global__ void k_import_spectra(const float2* __restrict__ Gx, const float2* __
 template <int M>
 struct P2Cfg;
 template <>
-struct P2Cfg<256> { static constexpr int RY = 16, NW = 8; };
+struct P2Cfg<256> { static constexpr int RY = 16, NW = 16; };
 template <>
-struct P2Cfg<512> { static constexpr int RY = 16, NW = 8; };
+struct P2Cfg<512> { static constexpr int RY = 16, NW = 16; };
 template <>
 struct P2Cfg<1024> { static constexpr int RY = 8, NW = 8; };
 template <>

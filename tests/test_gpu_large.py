@@ -1,0 +1,220 @@
+"""Large images, graph replay, autograd pairing and ragged batches (GPU).
+
+Large images are checked against the FP64 oracle on sampled windows (the
+oracle runs on a zero-padded sub-image that holds every tap the window reads)
+and through full-size dot-product identities.
+"""
+import numpy as np
+import pytest
+
+import shrinkage_full_tests_amd as srk
+from shrinkage_full_tests_amd import patterns
+
+pytestmark = pytest.mark.gpu
+
+TOL_REL, TOL_ABS = 1e-3, 1e-5
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def assert_close(a, n, what=""):
+    err = np.abs(a - n)
+    lim = np.maximum(TOL_ABS, TOL_REL * np.maximum(np.abs(a), np.abs(n)))
+    assert not (err > lim).any(), f"{what}: max err {err.max():.3e}, max|ref| {np.abs(n).max():.3e}"
+
+
+def crop(t, X0, Y0, S):
+    """[S, S] float64 copy of t[X0:X0+S, Y0:Y0+S] (t: [n, n] tensor), zero outside."""
+    n = t.shape[0]
+    out = np.zeros((S, S))
+    a0, a1 = max(X0, 0), min(X0 + S, n)
+    b0, b1 = max(Y0, 0), min(Y0 + S, n)
+    if a0 < a1 and b0 < b1:
+        out[a0 - X0:a1 - X0, b0 - Y0:b1 - Y0] = t[a0:a1, b0:b1].double().cpu().numpy()
+    return out
+
+
+def check_forward_window(orc, f, g, h, e, Kx, Ky, ox0, oy0, w):
+    """h[ox0:ox0+w, oy0:oy0+w] (valid coordinates) against the oracle."""
+    S = w + 2 * e + 2
+    X0, Y0 = ox0, oy0   # sub-image output 0 (pixel e) is image output ox0 (pixel ox0 + e)
+    ref = orc.forward_window(crop(f, X0, Y0, S), crop(g, X0, Y0, S), e, Kx, Ky, 0, 0, w, w)
+    got = h[ox0:ox0 + w, oy0:oy0 + w].double().cpu().numpy()
+    assert_close(got, ref, f"h window at ({ox0},{oy0})")
+
+
+def check_adjoint_window(orc, f, g, v, lf, lg, e, Kx, Ky, qx0, qy0, w):
+    pad = 2 * e + 2
+    X0, Y0, S = qx0 - pad, qy0 - pad, w + 2 * pad
+    fs, gs = crop(f, X0, Y0, S), crop(g, X0, Y0, S)
+    vs = crop(v, X0, Y0, S - 2 * e)   # valid-region coordinates shift the same way
+    a, b = orc.adjoint_window(vs, fs, gs, e, Kx, Ky, pad, pad, w, w)
+    assert_close(lf[qx0:qx0 + w, qy0:qy0 + w].double().cpu().numpy(), a,
+                 f"lambda_f window at ({qx0},{qy0})")
+    assert_close(lg[qx0:qx0 + w, qy0:qy0 + w].double().cpu().numpy(), b,
+                 f"lambda_g window at ({qx0},{qy0})")
+
+
+def _dot(a, b):
+    return float((a.double() * b.double()).sum())
+
+
+@pytest.mark.parametrize("N,kw,M", [(4096, 61, 512), (16384, 121, 1024)])
+def test_large_image_windows_and_dot_products(orc, N, kw, M):
+    torch = _torch()
+    e = (kw - 1) // 2
+    W = N - 2 * e
+    sigma = e / 3.0
+    gx, gy, hx, hy = srk.kernel_spectra_srsrk(kw, sigma, M, with_spatial=True)
+    Kx, Ky = hx.numpy().astype(np.float64), hy.numpy().astype(np.float64)
+    f = patterns.grf_torch(1, N, seed=3000)
+    g = patterns.grf_torch(1, N, seed=4000)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(11)
+    v = torch.randn((1, W, W), generator=gen, device="cuda")
+    ws = srk.Workspace(N, W, e, M)
+    h = srk.forward_batched(g, f, gx, gy, e, 1.0, ws)
+    lf, lg = srk.adjoint_batched(v, g, f, gx, gy, e, 1.0, ws)
+    torch.cuda.synchronize()
+    w = 40
+    S = M - 2 * e   # forward tile stride: probe windows across tile seams
+    for (ox0, oy0) in ((0, 0), (S - w // 2, S - w // 2), (W // 2, W // 3), (W - w, W - w)):
+        check_forward_window(orc, f[0], g[0], h[0], e, Kx, Ky, ox0, oy0, w)
+    Sa = M - 2 * e - 2
+    for (qx0, qy0) in ((0, 0), (Sa - w // 2, 2 * Sa - w // 2), (N // 2, N // 5), (N - w, N - w)):
+        check_adjoint_window(orc, f[0], g[0], v[0], lf[0], lg[0], e, Kx, Ky, qx0, qy0, w)
+    # full-size adjoint identities
+    df = patterns.grf_torch(1, N, seed=5000)
+    h_df = srk.forward_batched(g, df, gx, gy, e, 1.0, ws)
+    a1, b1 = _dot(h_df, v), _dot(df, lf)
+    del h_df
+    dg = patterns.grf_torch(1, N, seed=6000)
+    h_dg = srk.forward_batched(dg, f, gx, gy, e, 1.0, ws)
+    a2, b2 = _dot(h_dg, v), _dot(dg, lg)
+    assert abs(a1 - b1) <= 1e-4 * abs(a1), (a1, b1)
+    assert abs(a2 - b2) <= 1e-4 * abs(a2), (a2, b2)
+
+
+def test_hipgraph_replay_matches_eager():
+    torch = _torch()
+    N, kw, M, B = 512, 121, 512, 6
+    e = 60
+    W = N - 2 * e
+    gx, gy = srk.kernel_spectra_srsrk(kw, 20.0, M)
+    E = patterns.grf_torch(B, N, seed=9)
+    v = torch.randn((B, W, W), device="cuda")
+    ws = srk.Workspace(N, W, e, M, B_max=B)
+    R = torch.empty((B, W, W), device="cuda")
+    lf = torch.empty((B, N, N), device="cuda")
+    lg = torch.empty((B, N, N), device="cuda")
+
+    def step():
+        srk.srkTrainingForward(E, E, gx, gy, N, W, e, 1.0, B, ws, R)
+        srk.adjoint_batched(v, E, E, gx, gy, e, 1.0, ws, lf, lg)
+
+    step()
+    torch.cuda.synchronize()
+    ref = [R.clone(), lf.clone(), lg.clone()]
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            step()
+    torch.cuda.current_stream().wait_stream(s)
+    for it in range(3):
+        R.zero_(); lf.zero_(); lg.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for got, want in zip((R, lf, lg), ref):
+            assert torch.equal(got, want), f"replay {it} differs from eager"
+        _ = (E * 2).sum().item()   # eager work between replays
+
+
+def test_autograd_function_matches_adjoint():
+    torch = _torch()
+    N, kw, M, B = 256, 31, 256, 2
+    e = 15
+    W = N - 2 * e
+    gx, gy = srk.kernel_spectra_srsrk(kw, 5.0, M)
+    ws = srk.Workspace(N, W, e, M, B_max=B)
+    f = patterns.grf_torch(B, N, seed=1).requires_grad_(True)
+    g = patterns.grf_torch(B, N, seed=2).requires_grad_(True)
+    v = torch.randn((B, W, W), device="cuda")
+    h = srk.ShrinkageFunction.apply(f, g, gx, gy, e, 1.0, ws)
+    gf, gg = torch.autograd.grad((h * v).sum(), (f, g))
+    lf, lg = srk.adjoint_batched(v, g.detach(), f.detach(), gx, gy, e, 1.0, ws)
+    assert torch.equal(gf, lf) and torch.equal(gg, lg)
+    h2 = srk.forward_batched(g.detach(), f.detach(), gx, gy, e, 1.0, ws)
+    assert torch.equal(h.detach(), h2)
+
+
+def test_ragged_groups_in_one_graph(orc):
+    """A ragged list (N in {1024, 2048}, KW in {31, 61}) run as uniform groups,
+    forward + adjoint + tangent-adjoint captured in one graph; checked on
+    windows against the oracle and by dot products."""
+    torch = _torch()
+    rng = np.random.default_rng(4242)
+    inst = [(int(rng.choice([1024, 2048])), int(rng.choice([31, 61]))) for _ in range(6)]
+    groups = {}
+    for i, key in enumerate(inst):
+        groups.setdefault(key, []).append(i)
+    M = 512
+    state = []
+    for (N, kw), idx in groups.items():
+        e = (kw - 1) // 2
+        W = N - 2 * e
+        B = len(idx)
+        gx, gy, hx, hy = srk.kernel_spectra_srsrk(kw, e / 3.0, M, with_spatial=True)
+        ws = srk.Workspace(N, W, e, M, B_max=B)
+        f = patterns.grf_torch(B, N, seed=100 + N + kw)
+        g = patterns.grf_torch(B, N, seed=200 + N + kw)
+        v = torch.randn((B, W, W), device="cuda")
+        df = patterns.grf_torch(B, N, seed=300 + N + kw)
+        dg = patterns.grf_torch(B, N, seed=400 + N + kw)
+        dv = torch.randn((B, W, W), device="cuda")
+        out = dict(h=torch.empty((B, W, W), device="cuda"),
+                   lf=torch.empty((B, N, N), device="cuda"), lg=torch.empty((B, N, N), device="cuda"),
+                   tf=torch.empty((B, N, N), device="cuda"), tg=torch.empty((B, N, N), device="cuda"))
+        state.append(dict(N=N, kw=kw, e=e, W=W, gx=gx, gy=gy, ws=ws, f=f, g=g, v=v, df=df, dg=dg,
+                          dv=dv, Kx=hx.numpy().astype(np.float64), Ky=hy.numpy().astype(np.float64),
+                          **out))
+
+    def run_all():
+        for s in state:
+            srk.forward_batched(s["g"], s["f"], s["gx"], s["gy"], s["e"], 1.0, s["ws"], s["h"])
+            srk.adjoint_batched(s["v"], s["g"], s["f"], s["gx"], s["gy"], s["e"], 1.0, s["ws"],
+                                s["lf"], s["lg"])
+            srk.tangent_adjoint_batched(s["f"], s["g"], s["df"], s["dg"], s["v"], s["dv"], s["gx"],
+                                        s["gy"], s["e"], 1.0, s["ws"], s["tf"], s["tg"])
+
+    run_all()
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    st = torch.cuda.Stream()
+    st.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(st):
+        with torch.cuda.graph(graph, stream=st):
+            run_all()
+    torch.cuda.current_stream().wait_stream(st)
+    eager = [[s[k].clone() for k in ("h", "lf", "lg", "tf", "tg")] for s in state]
+    graph.replay()
+    torch.cuda.synchronize()
+    for s, ref in zip(state, eager):
+        for k, r in zip(("h", "lf", "lg", "tf", "tg"), ref):
+            assert torch.equal(s[k], r), k
+    for s in state:
+        e, W = s["e"], s["W"]
+        check_forward_window(orc, s["f"][0], s["g"][0], s["h"][0], e, s["Kx"], s["Ky"], W // 2, 7, 32)
+        check_adjoint_window(orc, s["f"][0], s["g"][0], s["v"][0], s["lf"][0], s["lg"][0], e,
+                             s["Kx"], s["Ky"], s["N"] - 40, 300, 32)
+        # tangent-adjoint: bilinear, so it equals adjoint(dv; f, g) + adjoint(v; df, dg)
+        a1, b1 = srk.adjoint_batched(s["dv"], s["g"], s["f"], s["gx"], s["gy"], e, 1.0, s["ws"])
+        a2, b2 = srk.adjoint_batched(s["v"], s["dg"], s["df"], s["gx"], s["gy"], e, 1.0, s["ws"])
+        for got, want in ((s["tf"], a1 + a2), (s["tg"], b1 + b2)):
+            scale = float(want.abs().max())
+            assert float((got - want).abs().max()) <= 1e-5 * scale + 1e-6

@@ -230,3 +230,19 @@ def test_cd_upsample_matches_direct_sum():
             assert abs(up[b, m] - ref) < 1e-12
     np.testing.assert_allclose(h.sum(axis=1), 1.0, atol=1e-12)
     np.testing.assert_allclose(up[:, ::L], s, atol=1e-9)   # phase 0 is the identity
+
+
+def test_window_forms_match_whole_image(orc):
+    N, kw, e = 44, 11, 6
+    Kx, Ky = _kernel(orc, kw)
+    f, g = _rand_fields(N, 77)
+    W = N - 2 * e
+    v = np.random.default_rng(6).standard_normal((W, W))
+    h = orc.forward(f, g, e, Kx, Ky, s=1.3)
+    np.testing.assert_allclose(orc.forward_window(f, g, e, Kx, Ky, 3, 5, 10, 7, s=1.3),
+                               h[3:13, 5:12], rtol=1e-12, atol=1e-14)
+    lf, lg = orc.adjoint(v, f, g, e, Kx, Ky, s=1.3)
+    for (x0, y0, wx, wy) in ((0, 0, 9, 8), (4, 30, 12, 14), (35, 36, 9, 8)):
+        a, b = orc.adjoint_window(v, f, g, e, Kx, Ky, x0, y0, wx, wy, s=1.3)
+        np.testing.assert_allclose(a, lf[x0:x0 + wx, y0:y0 + wy], rtol=1e-11, atol=1e-14)
+        np.testing.assert_allclose(b, lg[x0:x0 + wx, y0:y0 + wy], rtol=1e-11, atol=1e-14)
