@@ -76,10 +76,13 @@ def main():
     ap.add_argument("--chunk", type=int, default=0, help="tiles per chunk (0 = auto)")
     ap.add_argument("--scratch-mb", type=int, default=0)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--precision", default="both", choices=("fp32", "mixed_fp16", "both"))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--large-n", type=int, default=16384)
-    ap.add_argument("--large-m", type=int, default=1024)
+    ap.add_argument("--large-m", type=int, default=512)
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="run one plain device copy of this size first (PMC traffic calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -101,6 +104,14 @@ def main():
     M = args.m or N
     dx = 1.0
 
+    if args.calib_copy_mb:
+        n = args.calib_copy_mb * (1 << 20) // 4
+        src = torch.ones(n, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     gx, gy = srk.kernel_spectra_srsrk(kw, sigma, M)
     E = patterns.grf_torch(B, N, seed=1000)
     gen = torch.Generator(device="cuda")
@@ -112,6 +123,8 @@ def main():
     ws = srk.Workspace(N, W, e, M, B_max=B, scratch_mb=args.scratch_mb)
     if args.chunk:
         ws.set_chunk(args.chunk)
+    stream = torch.cuda.current_stream()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
     def fwd():
         if M == N:
@@ -126,46 +139,61 @@ def main():
         fwd()
         adj()
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-
-    graph = None
-    if not args.no_graph:
-        graph = torch.cuda.CUDAGraph()
-        s = torch.cuda.Stream()
-        s.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(s):
-            with torch.cuda.graph(graph, stream=s):
-                step()
-        torch.cuda.current_stream().wait_stream(s)
-        graph.replay()
+    def measure(precision):
+        """(ms per step, ms fwd, ms adj, hipgraph used) for one precision form."""
+        ws.set_precision(precision)
+        for _ in range(args.warmup):
+            step()
         torch.cuda.synchronize()
-
-    run = graph.replay if graph is not None else step
-    stream = torch.cuda.current_stream()
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    start.record(stream)
-    for _ in range(args.steps):
-        run()
-    stop.record(stream)
-    torch.cuda.synchronize()
-    ms = start.elapsed_time(stop) / args.steps
-
-    # forward / adjoint split (eager, same stream)
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-    reps = max(1, min(3, args.steps))
-    t_f = t_a = 0.0
-    for _ in range(reps):
-        ev[0].record(stream)
-        fwd()
-        ev[1].record(stream)
-        adj()
-        ev[2].record(stream)
+        graph = None
+        if not args.no_graph:
+            graph = torch.cuda.CUDAGraph()
+            s = torch.cuda.Stream()
+            s.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(s):
+                with torch.cuda.graph(graph, stream=s):
+                    step()
+            torch.cuda.current_stream().wait_stream(s)
+            graph.replay()
+            torch.cuda.synchronize()
+        run = graph.replay if graph is not None else step
         torch.cuda.synchronize()
-        t_f += ev[0].elapsed_time(ev[1]) / reps
-        t_a += ev[1].elapsed_time(ev[2]) / reps
+        start.record(stream)
+        for _ in range(args.steps):
+            run()
+        stop.record(stream)
+        torch.cuda.synchronize()
+        ms_ = start.elapsed_time(stop) / args.steps
+        # forward / adjoint split (eager, same stream)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        reps = max(1, min(3, args.steps))
+        tf = ta = 0.0
+        for _ in range(reps):
+            ev[0].record(stream)
+            fwd()
+            ev[1].record(stream)
+            adj()
+            ev[2].record(stream)
+            torch.cuda.synchronize()
+            tf += ev[0].elapsed_time(ev[1]) / reps
+            ta += ev[1].elapsed_time(ev[2]) / reps
+        del graph
+        return ms_, tf, ta
+
+    mixed = None
+    if args.precision in ("mixed_fp16", "both") and M <= 1024:
+        mms, mtf, mta = measure("mixed_fp16")
+        mixed = {"value": round(B * N * N / (mms / 1e3) / 1e6, 1), "unit": "Mpix/s",
+                 "ms_per_step": round(mms, 3), "ms_fwd": round(mtf, 3), "ms_adj": round(mta, 3),
+                 "dtype": "f16 MFMA operands, f32 accumulate",
+                 "tolerance": "tol_rel 1e-2, tol_abs 1e-3 vs FP64 (tests/test_gpu_mixed.py)"}
+    if args.precision == "mixed_fp16" and mixed is not None:
+        ms, t_f, t_a = mms, mtf, mta
+    else:
+        ms, t_f, t_a = measure("fp32")
+    if mixed is not None:
+        mixed["speedup_vs_fp32"] = round(ms / mixed["ms_per_step"], 3)
+    graph = not args.no_graph
 
     value = B * N * N / (ms / 1e3) / 1e6
     nbytes = alg_bytes(N, W, B, True)
@@ -191,19 +219,29 @@ def main():
             srk.forward_batched(gL, fL, gxl, gyl, e, dx, wsL, hL)
             srk.adjoint_batched(vL, gL, fL, gxl, gyl, e, dx, wsL, lfL, lgL)
 
-        stepL()
-        torch.cuda.synchronize()
-        start.record(stream)
-        for _ in range(3):
+        def time_large(precision):
+            wsL.set_precision(precision)
             stepL()
-        stop.record(stream)
-        torch.cuda.synchronize()
-        msL = start.elapsed_time(stop) / 3
+            torch.cuda.synchronize()
+            start.record(stream)
+            for _ in range(3):
+                stepL()
+            stop.record(stream)
+            torch.cuda.synchronize()
+            return start.elapsed_time(stop) / 3
+
         bL = alg_bytes(NL, WL, 1, False)
+        msL = time_large("fp32")
         secondary = {"workload": f"fwd+adj N={NL} B=1 KW={kw} f!=g, M={ML}",
                      "value": round(NL * NL / (msL / 1e3) / 1e6, 1), "unit": "Mpix/s",
                      "ms_per_step": round(msL, 3),
                      "hbm_frac": round(bL / (msL / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+        if args.precision != "fp32" and ML <= 1024:
+            msM = time_large("mixed_fp16")
+            secondary["mixed_fp16"] = {"value": round(NL * NL / (msM / 1e3) / 1e6, 1),
+                                       "ms_per_step": round(msM, 3),
+                                       "hbm_frac": round(bL / (msM / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                                       "speedup_vs_fp32": round(msL / msM, 3)}
 
     cpu = None if args.no_cpu else cpu_baseline(N, kw, sigma, e)
 
@@ -224,7 +262,8 @@ def main():
                                f"N={N} B={B} KW={kw} erosion={e} W={W} E=I, FP32",
                    "N": N, "B": B, "KW": kw, "erosion": e, "W": W, "M": M,
                    "tiles_per_chunk_fwd": info["chunk_fwd"], "tiles_per_chunk_adj": info["chunk_adj"],
-                   "hipgraph": graph is not None},
+                   "hipgraph": graph,
+                   "precision": "fp32" if args.precision != "mixed_fp16" else "mixed_fp16"},
         "ms_fwd": round(t_f, 3),
         "ms_adj": round(t_a, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -232,6 +271,7 @@ def main():
                      "traffic": None,
                      "scope": "one fwd+adj step (all passes), algorithmic bytes "
                               f"16N^2+8W^2 per image = {nbytes / 1e9:.2f} GB"},
+        "mixed_fp16": mixed,
         "secondary": secondary,
         "cpu_baseline": cpu,
     }

@@ -54,7 +54,9 @@ enum {
 
 /* precision of the tile transforms */
 enum {
-    SRK_PREC_FP32 = 0
+    SRK_PREC_FP32 = 0,        /* FP32 radix-4/8/16 transforms on the vector ALUs     */
+    SRK_PREC_MIXED_FP16 = 1   /* DFT stages as FP16 MFMA products, FP32 accumulation,
+                                 FP32 twiddles, per-line power-of-two scaling; M <= 1024 */
 };
 
 typedef struct srk_workspace_s* srk_workspace_t;
@@ -103,6 +105,8 @@ int srkWorkspaceCreate(int N, int W, int erosion, int M, int B_max, size_t scrat
 int srkWorkspaceDestroy(srk_workspace_t ws);   /* replaces SrkTrainWorkspace::release */
 size_t srkWorkspaceBytes(srk_workspace_t ws);  /* device bytes held */
 int srkWorkspaceSetChunk(srk_workspace_t ws, int tiles_per_chunk); /* 0 = automatic */
+/* SRK_PREC_FP32 (default) or SRK_PREC_MIXED_FP16 for every later call on ws */
+int srkWorkspaceSetPrecision(srk_workspace_t ws, int precision);
 int srkWorkspaceInfo(srk_workspace_t ws, int* M, int* fwd_tiles_per_image,
                      int* adj_tiles_per_image, int* chunk_fwd, int* chunk_adj);
 

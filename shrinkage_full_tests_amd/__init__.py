@@ -51,11 +51,15 @@ class Workspace:
     """Opaque device workspace for images of size N, erosion e, batches <= B_max,
     FFT/tile size M (the size of the spectra handed to the calls)."""
 
-    def __init__(self, N: int, W: int, erosion: int, M: int, B_max: int = 1, scratch_mb: int = 0):
+    def __init__(self, N: int, W: int, erosion: int, M: int, B_max: int = 1, scratch_mb: int = 0,
+                 precision="fp32"):
         h = ctypes.c_void_p()
         call("srkWorkspaceCreate", N, W, erosion, M, B_max, scratch_mb, ctypes.byref(h))
         self.handle = h.value
         self.N, self.W, self.erosion, self.M, self.B_max = N, W, erosion, M, B_max
+        self.precision = "fp32"
+        if precision != "fp32":
+            self.set_precision(precision)
 
     def close(self):
         if self.handle:
@@ -80,6 +84,12 @@ class Workspace:
 
     def set_chunk(self, tiles: int) -> None:
         call("srkWorkspaceSetChunk", self.handle, int(tiles))
+
+    def set_precision(self, precision) -> None:
+        """"fp32" (default) or "mixed_fp16" (FP16 MFMA DFT stages, FP32 accumulation)."""
+        code = {"fp32": 0, "mixed_fp16": 1}.get(precision, precision)
+        call("srkWorkspaceSetPrecision", self.handle, int(code))
+        self.precision = precision
 
     def info(self) -> dict:
         v = [ctypes.c_int() for _ in range(5)]

@@ -211,6 +211,7 @@ bool adj_tiles(int N, int e, int M, std::vector<srk::Tile>& t, int& S2)
 struct srk_workspace_s {
     int N = 0, W = 0, e = 0, M = 0, B_max = 0;
     int chunk_override = 0;
+    int precision = SRK_PREC_FP32;
     srk::Tile* d_ftiles = nullptr;
     srk::Tile* d_atiles = nullptr;
     int nft = 0, nat = 0, S2f = 0, S2a = 0;
@@ -309,7 +310,7 @@ int run_fwd(srk_workspace_s* w, srk::Op op, srk::EngineArgs a, int B, hipStream_
     a.halo2 = 0;
     const int ntiles = B * w->nft;
     const int chunk = chunk_for(w, a.np1, a.np2, a.S2, ntiles);
-    return hip_status(srk::engine_run(op, w->M, a, ntiles, chunk, s));
+    return hip_status(srk::engine_run(op, w->M, w->precision, a, ntiles, chunk, s));
 }
 
 int run_adj(srk_workspace_s* w, srk::Op op, srk::EngineArgs a, int B, hipStream_t s)
@@ -325,7 +326,7 @@ int run_adj(srk_workspace_s* w, srk::Op op, srk::EngineArgs a, int B, hipStream_
     a.halo2 = 1;
     const int ntiles = B * w->nat;
     const int chunk = chunk_for(w, a.np1, a.np2, a.S2, ntiles);
-    return hip_status(srk::engine_run(op, w->M, a, ntiles, chunk, s));
+    return hip_status(srk::engine_run(op, w->M, w->precision, a, ntiles, chunk, s));
 }
 
 }  // namespace
@@ -432,10 +433,30 @@ int srkWorkspaceCreate(int N, int W, int erosion, int M, int B_max, size_t scrat
     cmax = std::min<size_t>(cmax, (size_t)B_max * std::max(w->nft, w->nat));
     w->x1_bytes = cmax * x1_per;
     w->x2_bytes = cmax * x2_per;
-    std::vector<float2> tw(M);
+    // twiddles exp(-2 pi i k / M), followed by the per-lane FP16 hi/lo tiles of
+    // the 16-point DFT used by the matrix-core transforms (256 x 8 halves).
+    std::vector<float2> tw(M + 512);
     for (int k = 0; k < M; ++k) {
         const double a = -2.0 * M_PI * k / M;
         tw[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    {
+        _Float16* tiles = reinterpret_cast<_Float16*>(tw.data() + M);
+        for (int lane = 0; lane < 64; ++lane)
+            for (int p = 0; p < 2; ++p)
+                for (int nb = 0; nb < 2; ++nb)
+                    for (int e = 0; e < 8; ++e) {
+                        const int n = lane & 15, q = lane >> 4;
+                        const int r = 8 * (q & 1) + e;
+                        const bool in_im = q >= 2, out_im = nb == 1;
+                        const double th = 2.0 * M_PI * ((r * n) & 15) / 16.0;
+                        // Re X = sum a cos + b sin ; Im X = sum b cos - a sin
+                        const double v = in_im ? (out_im ? std::cos(th) : std::sin(th))
+                                               : (out_im ? -std::sin(th) : std::cos(th));
+                        const _Float16 hi = (_Float16)v;
+                        const _Float16 lo = (_Float16)(v - (double)hi);
+                        tiles[(((lane * 2 + p) * 2 + nb) * 8) + e] = p == 0 ? hi : lo;
+                    }
     }
     int st = SRK_OK;
     auto alloc = [&](void** p, size_t bytes) {
@@ -448,7 +469,7 @@ int srkWorkspaceCreate(int N, int W, int erosion, int M, int B_max, size_t scrat
     alloc((void**)&w->X1, w->x1_bytes);
     alloc((void**)&w->X2, w->x2_bytes);
     alloc((void**)&w->GT, (size_t)2 * (M / 2 + 1) * M * sizeof(float2));
-    alloc((void**)&w->tw, (size_t)M * sizeof(float2));
+    alloc((void**)&w->tw, tw.size() * sizeof(float2));
     if (st == SRK_OK)
         st = hip_status(hipMemcpy(w->d_ftiles, fb.data(), fb.size() * sizeof(srk::Tile), hipMemcpyHostToDevice));
     if (st == SRK_OK)
@@ -479,6 +500,21 @@ int srkWorkspaceSetChunk(srk_workspace_t w, int tiles)
     if (tiles < 0) return SRK_ERR_INVALID_ARG;
     w->chunk_override = tiles;
     return SRK_OK;
+}
+
+int srkWorkspaceSetPrecision(srk_workspace_t w, int precision)
+{
+    if (!w) return SRK_ERR_NULL_PTR;
+    if (precision == SRK_PREC_FP32) {
+        w->precision = precision;
+        return SRK_OK;
+    }
+    if (precision == SRK_PREC_MIXED_FP16) {
+        if (w->M > 1024) return SRK_ERR_UNSUPPORTED_SIZE;
+        w->precision = precision;
+        return SRK_OK;
+    }
+    return SRK_ERR_INVALID_ARG;
 }
 
 int srkWorkspaceInfo(srk_workspace_t w, int* M, int* fwd_tiles, int* adj_tiles, int* chunk_fwd,

@@ -10,6 +10,7 @@
 //   tangent-adj  tan_tanadj/srk_tangent_adjoint.cuh:100-159
 #include "srk_engine.hpp"
 #include "srk_fft.hpp"
+#include "srk_fft_mfma.hpp"
 
 #include <algorithm>
 
@@ -26,8 +27,8 @@ template <int M>
 __device__ __forceinline__ void unpack_store(const float2* line, float2* P, float2* Q, int lane)
 {
     for (int k = lane; k <= M / 2; k += kWave) {
-        const float2 z = line[k];
-        const float2 zm = line[(M - k) & (M - 1)];
+        const float2 z = line[lslot(k)];
+        const float2 zm = line[lslot((M - k) & (M - 1))];
         P[k] = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
         Q[k] = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
     }
@@ -42,9 +43,9 @@ __device__ __forceinline__ void zero_store(float2* P, int lane)
 // Inverse transform of two Hermitian half spectra P, Q (either may be null for
 // zero) packed as Z = P + iQ; afterwards line[y] = p(y) + i q(y).  The DC and
 // Nyquist bins are taken as real, as a real inverse transform does.
-template <int M>
-__device__ __forceinline__ void ifft_pair(float2* line, const float2* P, const float2* Q,
-                                          const float2* tw, int lane)
+template <int M, class PL>
+__device__ __forceinline__ void ifft_pair(const PL& FP, float2* line, const float2* P,
+                                          const float2* Q, int lane)
 {
     constexpr int V = M / kWave;
     float2 v[V];
@@ -60,7 +61,7 @@ __device__ __forceinline__ void ifft_pair(float2* line, const float2* P, const f
         }
         v[i] = k <= M / 2 ? make_float2(p.x - q.y, p.y + q.x) : make_float2(p.x + q.y, q.x - p.y);
     }
-    fft_from_regs<M, +1>(line, v, tw, lane);
+    fft_from_regs<M, +1>(FP, line, v, lane);
 }
 
 __device__ __forceinline__ float grad_x(const float* f, int N, int r, int c)
@@ -86,13 +87,15 @@ __device__ __forceinline__ float cot_at(const float* v, int N, int e, int W, int
 // ---------------------------------------------------------------------------
 // P1: real FFT along y of the convolved fields; one wavefront per column pair.
 // ---------------------------------------------------------------------------
-template <int M>
+template <int M, class PL>
 __global__ __launch_bounds__(256) void k_p1_cols(EngineArgs a, int nf)
 {
     constexpr int V = M / kWave;
     constexpr int HP = spec_pitch(M);
-    __shared__ float2 lds[4][M];
+    __shared__ float2 lds[4][line_slots(M)];
     const int lane = lane_id(), w = wave_id();
+    PL FP;
+    FP.load(a.tw, lane);
     const int slot = blockIdx.y;
     const Tile t = a.tiles[slot];
     const int N = a.N;
@@ -119,7 +122,7 @@ __global__ __launch_bounds__(256) void k_p1_cols(EngineArgs a, int nf)
             v[i].x = (va && rv) ? colA[row] : 0.f;
             v[i].y = (vb && rv) ? colB[row] : 0.f;
         }
-        fft_from_regs<M, -1>(line, v, a.tw, lane);
+        fft_from_regs<M, -1>(FP, line, v, lane);
         unpack_store<M>(line, outA, outB, lane);
     }
 }
@@ -128,12 +131,12 @@ __global__ __launch_bounds__(256) void k_p1_cols(EngineArgs a, int nf)
 // P1A: adjoint source a_d = v0 * d_d f0 (+ v1 * d_d f1) on the valid region,
 // transformed along y as a_x + i a_y; one wavefront per column.
 // ---------------------------------------------------------------------------
-template <int M>
+template <int M, class PL>
 __global__ __launch_bounds__(256) void k_p1_adj_src(EngineArgs a, int nf, int plane0)
 {
     constexpr int V = M / kWave;
     constexpr int HP = spec_pitch(M);
-    __shared__ float2 lds[4][M];
+    __shared__ float2 lds[4][line_slots(M)];
     const int lane = lane_id(), w = wave_id();
     const int slot = blockIdx.y;
     const Tile t = a.tiles[slot];
@@ -169,7 +172,9 @@ __global__ __launch_bounds__(256) void k_p1_adj_src(EngineArgs a, int nf, int pl
         v[i] = make_float2(ax, ay);
     }
     float2* line = lds[w];
-    fft_from_regs<M, -1>(line, v, a.tw, lane);
+    PL FP;
+    FP.load(a.tw, lane);
+    fft_from_regs<M, -1>(FP, line, v, lane);
     unpack_store<M>(line, outX, outY, lane);
 }
 
@@ -182,7 +187,7 @@ __device__ __forceinline__ void load_bands(float2* L, const float2* plane, int k
     constexpr int HP = spec_pitch(M);
     for (int idx = threadIdx.x; idx < M * RY; idx += blockDim.x) {
         const int x = idx / RY, r = idx - x * RY;
-        L[r * LP + x] = r < nrows ? plane[(long long)x * HP + k0 + r] : make_float2(0.f, 0.f);
+        L[r * LP + lslot(x)] = r < nrows ? plane[(long long)x * HP + k0 + r] : make_float2(0.f, 0.f);
     }
 }
 
@@ -196,7 +201,7 @@ __device__ __forceinline__ void store_bands(float2* plane, const float2* L, int 
         if (r < nrows) {
             const int x = lo + xi;
             plane[(long long)xi * HP + k0 + r] =
-                (x >= 0 && x < M) ? L[r * LP + x] : make_float2(0.f, 0.f);
+                (x >= 0 && x < M) ? L[r * LP + lslot(x)] : make_float2(0.f, 0.f);
         }
     }
 }
@@ -205,16 +210,18 @@ __device__ __forceinline__ void store_bands(float2* plane, const float2* L, int 
 // P2 conv: x-FFT of each convolved-field band, products with Gx and Gy,
 // inverse x-FFTs, crop; field fi -> X2 planes 2fi (u_x) and 2fi+1 (u_y).
 // ---------------------------------------------------------------------------
-template <int M, int RY, int NW>
+template <int M, int RY, int NW, class PL>
 __global__ __launch_bounds__(NW * kWave) void k_p2_conv(EngineArgs a, int nf)
 {
     constexpr int H = M / 2 + 1;
     constexpr int HP = spec_pitch(M);
-    constexpr int LP = M + 1;
+    constexpr int LP = line_slots(M) + 1;
     extern __shared__ float2 sm[];
     float2* A = sm;
     float2* Bs = sm + RY * LP;
     const int lane = lane_id(), w = wave_id();
+    PL FP;
+    FP.load(a.tw, lane);
     const int slot = blockIdx.y;
     const Tile t = a.tiles[slot];
     const int k0 = blockIdx.x * RY;
@@ -228,16 +235,16 @@ __global__ __launch_bounds__(NW * kWave) void k_p2_conv(EngineArgs a, int nf)
         for (int r = w; r < nrows; r += NW) {
             float2* la = A + r * LP;
             float2* lb = Bs + r * LP;
-            fft_in_lds<M, -1>(la, a.tw, lane);
+            fft_in_lds<M, -1>(FP, la, lane);
             const float2* gxr = a.GT + (long long)(k0 + r) * M;
             const float2* gyr = a.GT + (long long)(H + k0 + r) * M;
             for (int kx = lane; kx < M; kx += kWave) {
-                const float2 X = la[kx];
-                lb[kx] = cmul(X, gyr[kx]);
-                la[kx] = cmul(X, gxr[kx]);
+                const float2 X = la[lslot(kx)];
+                lb[lslot(kx)] = cmul(X, gyr[kx]);
+                la[lslot(kx)] = cmul(X, gxr[kx]);
             }
-            fft_in_lds<M, +1>(la, a.tw, lane);
-            fft_in_lds<M, +1>(lb, a.tw, lane);
+            fft_in_lds<M, +1>(FP, la, lane);
+            fft_in_lds<M, +1>(FP, lb, lane);
         }
         __syncthreads();
         float2* dstA = a.X2 + ((long long)slot * a.np2 + 2 * fi) * a.S2 * HP;
@@ -252,16 +259,18 @@ __global__ __launch_bounds__(NW * kWave) void k_p2_conv(EngineArgs a, int nf)
 // P2 corr: x-FFT of the adjoint sources, L = conj(Gx) A_x + conj(Gy) A_y,
 // inverse x-FFT, crop -> X2 plane planeOut.
 // ---------------------------------------------------------------------------
-template <int M, int RY, int NW>
+template <int M, int RY, int NW, class PL>
 __global__ __launch_bounds__(NW * kWave) void k_p2_corr(EngineArgs a, int planeIn, int planeOut)
 {
     constexpr int H = M / 2 + 1;
     constexpr int HP = spec_pitch(M);
-    constexpr int LP = M + 1;
+    constexpr int LP = line_slots(M) + 1;
     extern __shared__ float2 sm[];
     float2* A = sm;
     float2* Bs = sm + RY * LP;
     const int lane = lane_id(), w = wave_id();
+    PL FP;
+    FP.load(a.tw, lane);
     const int slot = blockIdx.y;
     const Tile t = a.tiles[slot];
     const int k0 = blockIdx.x * RY;
@@ -274,16 +283,16 @@ __global__ __launch_bounds__(NW * kWave) void k_p2_corr(EngineArgs a, int planeI
     for (int r = w; r < nrows; r += NW) {
         float2* la = A + r * LP;
         float2* lb = Bs + r * LP;
-        fft_in_lds<M, -1>(la, a.tw, lane);
-        fft_in_lds<M, -1>(lb, a.tw, lane);
+        fft_in_lds<M, -1>(FP, la, lane);
+        fft_in_lds<M, -1>(FP, lb, lane);
         const float2* gxr = a.GT + (long long)(k0 + r) * M;
         const float2* gyr = a.GT + (long long)(H + k0 + r) * M;
         for (int kx = lane; kx < M; kx += kWave) {
-            const float2 p = cmulc(la[kx], gxr[kx]);
-            const float2 q = cmulc(lb[kx], gyr[kx]);
-            la[kx] = cadd(p, q);
+            const float2 p = cmulc(la[lslot(kx)], gxr[kx]);
+            const float2 q = cmulc(lb[lslot(kx)], gyr[kx]);
+            la[lslot(kx)] = cadd(p, q);
         }
-        fft_in_lds<M, +1>(la, a.tw, lane);
+        fft_in_lds<M, +1>(FP, la, lane);
     }
     __syncthreads();
     store_bands<M, RY, LP>(a.X2 + ((long long)slot * a.np2 + planeOut) * a.S2 * HP, A, k0, nrows,
@@ -294,17 +303,19 @@ __global__ __launch_bounds__(NW * kWave) void k_p2_corr(EngineArgs a, int planeI
 // P3 forward / tangent: inverse y-FFT of (u_x + i u_y) per output column,
 // gradient stencil and combine, valid-region store.
 // ---------------------------------------------------------------------------
-template <int M>
+template <int M, class PL>
 __global__ __launch_bounds__(256) void k_p3_fwd(EngineArgs a, int nf)
 {
     constexpr int V = M / kWave;
     constexpr int HP = spec_pitch(M);
-    __shared__ float2 lds[4][M];
+    __shared__ float2 lds[4][line_slots(M)];
     const int lane = lane_id(), w = wave_id();
     const int slot = blockIdx.y;
     const Tile t = a.tiles[slot];
     const int xi = blockIdx.x * 4 + w;
     if (xi >= t.ox1 - t.ox0) return;
+    PL FP;
+    FP.load(a.tw, lane);
     const int N = a.N, e = a.e, W = a.W;
     const int gx = t.ox0 + xi;
     float2* line = lds[w];
@@ -314,12 +325,12 @@ __global__ __launch_bounds__(256) void k_p3_fwd(EngineArgs a, int nf)
     float* out = a.out0 + (long long)t.b * a.w_stride + (long long)(gx - e) * W;
 
     // u(g) for this column
-    ifft_pair<M>(line, base, base + pstride, a.tw, lane);
+    ifft_pair<M, PL>(FP, line, base, base + pstride, lane);
     float2 u0[V];
 #pragma unroll
     for (int j = 0; j < V; ++j) {
         const int gy = t.oy0 + lane + kWave * j;
-        u0[j] = gy < t.oy1 ? line[gy - t.p0y] : make_float2(0.f, 0.f);
+        u0[j] = gy < t.oy1 ? line[lslot(gy - t.p0y)] : make_float2(0.f, 0.f);
     }
     if (nf == 1) {
 #pragma unroll
@@ -343,12 +354,12 @@ __global__ __launch_bounds__(256) void k_p3_fwd(EngineArgs a, int nf)
     }
     // tangent: + u(dg) . grad f + u(g) . grad df
     const float* f1 = a.f1 + (long long)t.b * a.img_stride;
-    ifft_pair<M>(line, base + 2 * pstride, base + 3 * pstride, a.tw, lane);
+    ifft_pair<M, PL>(FP, line, base + 2 * pstride, base + 3 * pstride, lane);
 #pragma unroll
     for (int j = 0; j < V; ++j) {
         const int gy = t.oy0 + lane + kWave * j;
         if (gy < t.oy1) {
-            const float2 u1 = line[gy - t.p0y];
+            const float2 u1 = line[lslot(gy - t.p0y)];
             out[gy - e] = u1.x * grad_x(f0, N, gy, gx) + u1.y * grad_y(f0, N, gy, gx) +
                           u0[j].x * grad_x(f1, N, gy, gx) + u0[j].y * grad_y(f1, N, gy, gx);
         }
@@ -362,18 +373,20 @@ __global__ __launch_bounds__(256) void k_p3_fwd(EngineArgs a, int nf)
 //   lambda_g = inverse y-FFT of the correlation plane.
 // X2 column index of global column c is c - ox0 + 1.
 // ---------------------------------------------------------------------------
-template <int M>
+template <int M, class PL>
 __global__ __launch_bounds__(256) void k_p3_adj(EngineArgs a, int nf, int with_u, int planeL)
 {
     constexpr int V = M / kWave;
     constexpr int HP = spec_pitch(M);
-    __shared__ float2 lds[4][M];
+    __shared__ float2 lds[4][line_slots(M)];
     const int lane = lane_id(), w = wave_id();
     const int slot = blockIdx.y;
     const Tile t = a.tiles[slot];
     const int j0 = (blockIdx.x * 4 + w) * 2;
     const int cnt = t.ox1 - t.ox0;
     if (j0 >= cnt) return;
+    PL FP;
+    FP.load(a.tw, lane);
     const bool second = j0 + 1 < cnt;
     const int N = a.N, e = a.e, W = a.W;
     const int gx = t.ox0 + j0;
@@ -412,33 +425,33 @@ __global__ __launch_bounds__(256) void k_p3_adj(EngineArgs a, int nf, int with_u
             }
             const int px = 2 * fi, py = 2 * fi + 1;
             // u_x at gx-1, gx+1 -> x-stencil transpose at gx
-            ifft_pair<M>(line, col(px, -1), col(px, 1), a.tw, lane);
+            ifft_pair<M, PL>(FP, line, col(px, -1), col(px, 1), lane);
 #pragma unroll
             for (int j = 0; j < V; ++j) {
-                const float2 u = line[lane + kWave * j];
+                const float2 u = line[lslot(lane + kWave * j)];
                 acc0[j] += 0.5f * (u.x * cv[0][j] - u.y * cv[2][j]);
             }
             // u_x at gx, gx+2 -> x-stencil transpose at gx+1
-            ifft_pair<M>(line, col(px, 0), col(px, 2), a.tw, lane);
+            ifft_pair<M, PL>(FP, line, col(px, 0), col(px, 2), lane);
 #pragma unroll
             for (int j = 0; j < V; ++j) {
-                const float2 u = line[lane + kWave * j];
+                const float2 u = line[lslot(lane + kWave * j)];
                 acc1[j] += 0.5f * (u.x * cv[1][j] - u.y * cv[3][j]);
             }
             // u_y at gx, gx+1: w_y = u_y v formed in place, then the y-stencil
             // transpose reads the neighbouring rows (same wavefront, in order).
-            ifft_pair<M>(line, col(py, 0), col(py, 1), a.tw, lane);
+            ifft_pair<M, PL>(FP, line, col(py, 0), col(py, 1), lane);
 #pragma unroll
             for (int j = 0; j < V; ++j) {
                 const int y = lane + kWave * j;
-                const float2 u = line[y];
-                line[y] = make_float2(u.x * cv[1][j], u.y * cv[2][j]);
+                const float2 u = line[lslot(y)];
+                line[lslot(y)] = make_float2(u.x * cv[1][j], u.y * cv[2][j]);
             }
 #pragma unroll
             for (int j = 0; j < V; ++j) {
                 const int y = lane + kWave * j;
-                const float2 wm = y - 1 >= 0 ? line[y - 1] : make_float2(0.f, 0.f);
-                const float2 wp = y + 1 < M ? line[y + 1] : make_float2(0.f, 0.f);
+                const float2 wm = y - 1 >= 0 ? line[lslot(y - 1)] : make_float2(0.f, 0.f);
+                const float2 wp = y + 1 < M ? line[lslot(y + 1)] : make_float2(0.f, 0.f);
                 acc0[j] += 0.5f * (wm.x - wp.x);
                 acc1[j] += 0.5f * (wm.y - wp.y);
             }
@@ -454,13 +467,13 @@ __global__ __launch_bounds__(256) void k_p3_adj(EngineArgs a, int nf, int with_u
         }
     }
     // lambda_g
-    ifft_pair<M>(line, col(planeL, 0), col(planeL, 1), a.tw, lane);
+    ifft_pair<M, PL>(FP, line, col(planeL, 0), col(planeL, 1), lane);
     float* o1 = a.out1 + ib + (long long)gx * N;
 #pragma unroll
     for (int j = 0; j < V; ++j) {
         const int gy = t.oy0 + lane + kWave * j;
         if (gy < t.oy1) {
-            const float2 l = line[gy - t.p0y];
+            const float2 l = line[lslot(gy - t.p0y)];
             o1[gy] = l.x;
             if (second) o1[N + gy] = l.y;
         }
@@ -530,34 +543,48 @@ struct P2Cfg<1024> { static constexpr int RY = 8, NW = 8; };
 template <>
 struct P2Cfg<2048> { static constexpr int RY = 4, NW = 4; };
 
+template <class PL>
+struct IsMfmaPlan { static constexpr bool value = false; };
+template <int M>
+struct IsMfmaPlan<FftPlanMfma<M>> { static constexpr bool value = true; };
+
+// Wavefronts per P2 block: one LDS line per wavefront, except that the MFMA
+// plan holds more per-lane constants and runs two lines per wavefront at
+// M >= 512 so that a lane may use 256 registers.
+template <int M, class PL>
+struct P2Sel {
+    static constexpr int RY = P2Cfg<M>::RY;
+    static constexpr int NW = (IsMfmaPlan<PL>::value && M >= 512) ? P2Cfg<M>::NW / 2 : P2Cfg<M>::NW;
+};
+
 template <int M>
 constexpr size_t p2_lds()
 {
-    return 2ull * P2Cfg<M>::RY * (M + 1) * sizeof(float2);
+    return 2ull * P2Cfg<M>::RY * (line_slots(M) + 1) * sizeof(float2);
 }
 
-template <int M>
+template <int M, class PL>
 hipError_t prepare_kernels()
 {
     static bool done = false;
     static hipError_t st = hipSuccess;
     if (done) return st;
-    constexpr int RY = P2Cfg<M>::RY, NW = P2Cfg<M>::NW;
-    st = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_p2_conv<M, RY, NW>),
+    constexpr int RY = P2Sel<M, PL>::RY, NW = P2Sel<M, PL>::NW;
+    st = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_p2_conv<M, RY, NW, PL>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)p2_lds<M>());
     if (st == hipSuccess)
-        st = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_p2_corr<M, RY, NW>),
+        st = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_p2_corr<M, RY, NW, PL>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)p2_lds<M>());
     done = true;
     return st;
 }
 
-template <int M>
+template <int M, class PL>
 int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s)
 {
     constexpr int H = M / 2 + 1;
-    constexpr int RY = P2Cfg<M>::RY, NW = P2Cfg<M>::NW;
-    hipError_t st = prepare_kernels<M>();
+    constexpr int RY = P2Sel<M, PL>::RY, NW = P2Sel<M, PL>::NW;
+    hipError_t st = prepare_kernels<M, PL>();
     if (st != hipSuccess) return (int)st;
     const size_t lds2 = p2_lds<M>();
     const int cols_out = a0.S2 - 2 * a0.halo2;  // max output columns per tile
@@ -570,33 +597,33 @@ int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s)
         const dim3 b256(256), b2(NW * kWave);
         switch (op) {
         case Op::Forward:
-            k_p1_cols<M><<<g1, b256, 0, s>>>(a, 1);
-            k_p2_conv<M, RY, NW><<<g2, b2, lds2, s>>>(a, 1);
-            k_p3_fwd<M><<<g3f, b256, 0, s>>>(a, 1);
+            k_p1_cols<M, PL><<<g1, b256, 0, s>>>(a, 1);
+            k_p2_conv<M, RY, NW, PL><<<g2, b2, lds2, s>>>(a, 1);
+            k_p3_fwd<M, PL><<<g3f, b256, 0, s>>>(a, 1);
             break;
         case Op::Tangent:
-            k_p1_cols<M><<<g1, b256, 0, s>>>(a, 2);
-            k_p2_conv<M, RY, NW><<<g2, b2, lds2, s>>>(a, 2);
-            k_p3_fwd<M><<<g3f, b256, 0, s>>>(a, 2);
+            k_p1_cols<M, PL><<<g1, b256, 0, s>>>(a, 2);
+            k_p2_conv<M, RY, NW, PL><<<g2, b2, lds2, s>>>(a, 2);
+            k_p3_fwd<M, PL><<<g3f, b256, 0, s>>>(a, 2);
             break;
         case Op::Adjoint:
-            k_p1_cols<M><<<g1, b256, 0, s>>>(a, 1);
-            k_p1_adj_src<M><<<g1a, b256, 0, s>>>(a, 1, 1);
-            k_p2_conv<M, RY, NW><<<g2, b2, lds2, s>>>(a, 1);
-            k_p2_corr<M, RY, NW><<<g2, b2, lds2, s>>>(a, 1, 2);
-            k_p3_adj<M><<<g3a, b256, 0, s>>>(a, 1, 1, 2);
+            k_p1_cols<M, PL><<<g1, b256, 0, s>>>(a, 1);
+            k_p1_adj_src<M, PL><<<g1a, b256, 0, s>>>(a, 1, 1);
+            k_p2_conv<M, RY, NW, PL><<<g2, b2, lds2, s>>>(a, 1);
+            k_p2_corr<M, RY, NW, PL><<<g2, b2, lds2, s>>>(a, 1, 2);
+            k_p3_adj<M, PL><<<g3a, b256, 0, s>>>(a, 1, 1, 2);
             break;
         case Op::AdjointG:
-            k_p1_adj_src<M><<<g1a, b256, 0, s>>>(a, 1, 0);
-            k_p2_corr<M, RY, NW><<<g2, b2, lds2, s>>>(a, 0, 0);
-            k_p3_adj<M><<<g3a, b256, 0, s>>>(a, 1, 0, 0);
+            k_p1_adj_src<M, PL><<<g1a, b256, 0, s>>>(a, 1, 0);
+            k_p2_corr<M, RY, NW, PL><<<g2, b2, lds2, s>>>(a, 0, 0);
+            k_p3_adj<M, PL><<<g3a, b256, 0, s>>>(a, 1, 0, 0);
             break;
         case Op::TangentAdjoint:
-            k_p1_cols<M><<<g1, b256, 0, s>>>(a, 2);
-            k_p1_adj_src<M><<<g1a, b256, 0, s>>>(a, 2, 2);
-            k_p2_conv<M, RY, NW><<<g2, b2, lds2, s>>>(a, 2);
-            k_p2_corr<M, RY, NW><<<g2, b2, lds2, s>>>(a, 2, 4);
-            k_p3_adj<M><<<g3a, b256, 0, s>>>(a, 2, 1, 4);
+            k_p1_cols<M, PL><<<g1, b256, 0, s>>>(a, 2);
+            k_p1_adj_src<M, PL><<<g1a, b256, 0, s>>>(a, 2, 2);
+            k_p2_conv<M, RY, NW, PL><<<g2, b2, lds2, s>>>(a, 2);
+            k_p2_corr<M, RY, NW, PL><<<g2, b2, lds2, s>>>(a, 2, 4);
+            k_p3_adj<M, PL><<<g3a, b256, 0, s>>>(a, 2, 1, 4);
             break;
         }
         st = hipGetLastError();
@@ -607,17 +634,28 @@ int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s)
 
 }  // namespace
 
-int engine_run(Op op, int M, const EngineArgs& a, int ntiles, int chunk, hipStream_t s)
+int engine_run(Op op, int M, int prec, const EngineArgs& a, int ntiles, int chunk, hipStream_t s)
 {
     if (chunk < 1 || ntiles < 0) return (int)hipErrorInvalidValue;
     if (ntiles == 0) return 0;
-    switch (M) {
-    case 256: return run_impl<256>(op, a, ntiles, chunk, s);
-    case 512: return run_impl<512>(op, a, ntiles, chunk, s);
-    case 1024: return run_impl<1024>(op, a, ntiles, chunk, s);
-    case 2048: return run_impl<2048>(op, a, ntiles, chunk, s);
-    default: return (int)hipErrorInvalidValue;
+    if (prec == 0) {
+        switch (M) {
+        case 256: return run_impl<256, FftPlan<256>>(op, a, ntiles, chunk, s);
+        case 512: return run_impl<512, FftPlan<512>>(op, a, ntiles, chunk, s);
+        case 1024: return run_impl<1024, FftPlan<1024>>(op, a, ntiles, chunk, s);
+        case 2048: return run_impl<2048, FftPlan<2048>>(op, a, ntiles, chunk, s);
+        default: return (int)hipErrorInvalidValue;
+        }
     }
+    if (prec == 1) {
+        switch (M) {
+        case 256: return run_impl<256, FftPlanMfma<256>>(op, a, ntiles, chunk, s);
+        case 512: return run_impl<512, FftPlanMfma<512>>(op, a, ntiles, chunk, s);
+        case 1024: return run_impl<1024, FftPlanMfma<1024>>(op, a, ntiles, chunk, s);
+        default: return (int)hipErrorInvalidValue;
+        }
+    }
+    return (int)hipErrorInvalidValue;
 }
 
 size_t engine_p2_lds_bytes(int M)

@@ -69,8 +69,10 @@ enum class Op : int {
 };
 
 // Launch the passes of `op` for tiles [0, ntiles) of the tile table
-// args.tiles, chunk tiles at a time.  Returns a hipError_t.
-int engine_run(Op op, int M, const EngineArgs& args, int ntiles, int chunk, hipStream_t stream);
+// args.tiles, chunk tiles at a time.  prec 0: FP32 transforms; 1: FP16 MFMA
+// DFT stages with FP32 accumulation (M <= 1024).  Returns a hipError_t.
+int engine_run(Op op, int M, int prec, const EngineArgs& args, int ntiles, int chunk,
+               hipStream_t stream);
 
 // Transpose + scale reference-layout spectra (Ghat[k_x*(M/2+1)+k_y]) into GT.
 int engine_import_spectra(int M, const float2* Gx_ref, const float2* Gy_ref, float scale,

@@ -8,8 +8,15 @@
 // only the owning wavefront touches the line, so passes run in place with no
 // barrier (LDS operations of one wavefront complete in issue order).
 //
-// Twiddles come from a table tw[i] = exp(-2*pi*i*i/M) (i < M) built in double
-// precision on the host; the inverse transform uses conj(tw).  Transforms are
+// LDS line layout: element i lives at slot i + i/8 (one padding slot per
+// eight).  Without it the first two radix-8 passes write with a stride of 8 or
+// 64 complex values and every 64-bit store of a wavefront lands on the same
+// few banks.
+//
+// Twiddles: a lane's twiddles depend only on its butterfly index, which is the
+// same for every line, so each kernel loads them once (FftPlan::load) from a
+// table tw[i] = exp(-2*pi*i*i/M) built in double precision on the host and
+// keeps them in registers; the inverse uses their conjugates.  Transforms are
 // unnormalised in both directions.
 //
 // Radix plans (64 lanes): 256 = 4.4.4.4, 512 = 8.8.8, 1024 = 16.8.8,
@@ -22,6 +29,10 @@ namespace srk {
 
 constexpr int kWave = 64;
 
+// slot of element i in an LDS line, and slots per line of M elements
+__host__ __device__ constexpr int lslot(int i) { return i + (i >> 3); }
+__host__ __device__ constexpr int line_slots(int M) { return M + M / 8; }
+
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -33,7 +44,6 @@ __device__ __forceinline__ float2 cmulc(float2 a, float2 b)
 }
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 
 // cos / sin of 2*pi*j/16, j = 0..15 (exact constants of the 16th roots of unity)
 __device__ constexpr float kCos16[16] = {
@@ -105,8 +115,6 @@ __device__ __forceinline__ void dft(float2 (&x)[R])
 {
 #pragma unroll
     for (int i = 0; i < R; ++i) {
-        constexpr int dummy = 0;
-        (void)dummy;
         const int j = bitrev<R>(i);
         if (i < j) {
             float2 t = x[i];
@@ -117,11 +125,46 @@ __device__ __forceinline__ void dft(float2 (&x)[R])
     dit_stages<R, 2, DIR>(x);
 }
 
+// Per-lane twiddles of one pass.  For Ns <= 64 the twiddle index k = j mod Ns
+// of butterfly j = lane + 64 g does not depend on g, so one set is kept;
+// otherwise one set per g.  CACHE = false reads the table at use instead.
+template <int M, int R, int Ns, bool CACHE = true>
+struct PassTw {
+    static constexpr int NB = M / R;
+    static constexpr int G = NB / kWave;
+    static constexpr bool kShared = Ns <= kWave;
+    static constexpr int NG = (Ns == 1 || !CACHE) ? 0 : (kShared ? 1 : G);
+    float2 w[NG > 0 ? NG : 1][R - 1];
+    const float2* tab;
+
+    __device__ __forceinline__ void load(const float2* tw, int lane)
+    {
+        tab = tw;
+        if constexpr (NG > 0) {
+#pragma unroll
+            for (int gi = 0; gi < NG; ++gi) {
+                const int k = (lane + kWave * gi) & (Ns - 1);
+#pragma unroll
+                for (int r = 1; r < R; ++r) w[gi][r - 1] = tw[r * k * (M / (Ns * R))];
+            }
+        }
+    }
+    __device__ __forceinline__ float2 get(int g, int r, int lane) const
+    {
+        if constexpr (NG > 0) {
+            return w[kShared ? 0 : g][r - 1];
+        } else {
+            const int k = (lane + kWave * g) & (Ns - 1);
+            return tab[r * k * (M / (Ns * R))];
+        }
+    }
+};
+
 // One Stockham pass of radix R with span Ns over a line of M values.
 // FROM_REGS: operands come from v[] (element lane + 64*i in v[i]) instead of LDS.
-template <int M, int R, int Ns, int DIR, bool FROM_REGS>
+template <int M, int R, int Ns, int DIR, bool FROM_REGS, bool CACHE>
 __device__ __forceinline__ void stockham_pass(float2* __restrict__ line, const float2 (&v)[M / kWave],
-                                              const float2* __restrict__ tw, int lane)
+                                              const PassTw<M, R, Ns, CACHE>& T, int lane)
 {
     constexpr int NB = M / R;       // butterflies in the pass
     constexpr int G = NB / kWave;   // butterflies per lane
@@ -134,7 +177,7 @@ __device__ __forceinline__ void stockham_pass(float2* __restrict__ line, const f
             if constexpr (FROM_REGS)
                 x[g][r] = v[g + r * G];
             else
-                x[g][r] = line[lane + kWave * g + r * NB];
+                x[g][r] = line[lslot(lane + kWave * g + r * NB)];
         }
 #pragma unroll
     for (int g = 0; g < G; ++g) {
@@ -143,76 +186,107 @@ __device__ __forceinline__ void stockham_pass(float2* __restrict__ line, const f
         if constexpr (Ns > 1) {
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                float2 w = tw[r * k * (M / (Ns * R))];
+                const float2 w = T.get(g, r, lane);
                 x[g][r] = DIR < 0 ? cmul(x[g][r], w) : cmulc(x[g][r], w);
             }
         }
         dft<R, DIR>(x[g]);
         const int base = (j / Ns) * Ns * R + k;
 #pragma unroll
-        for (int r = 0; r < R; ++r) line[base + r * Ns] = x[g][r];
+        for (int r = 0; r < R; ++r) line[lslot(base + r * Ns)] = x[g][r];
     }
 }
 
 template <int M>
 struct FftPlan;
+
 template <>
 struct FftPlan<256> {
-    template <int DIR, bool REGS>
-    __device__ static void run(float2* line, const float2 (&v)[4], const float2* tw, int lane)
+    PassTw<256, 4, 1> t0;
+    PassTw<256, 4, 4> t1;
+    PassTw<256, 4, 16> t2;
+    PassTw<256, 4, 64> t3;
+    __device__ void load(const float2* tw, int lane)
     {
-        stockham_pass<256, 4, 1, DIR, REGS>(line, v, tw, lane);
-        stockham_pass<256, 4, 4, DIR, false>(line, v, tw, lane);
-        stockham_pass<256, 4, 16, DIR, false>(line, v, tw, lane);
-        stockham_pass<256, 4, 64, DIR, false>(line, v, tw, lane);
+        t0.load(tw, lane); t1.load(tw, lane); t2.load(tw, lane); t3.load(tw, lane);
+    }
+    template <int DIR, bool REGS>
+    __device__ void run(float2* line, const float2 (&v)[4], int lane) const
+    {
+        stockham_pass<256, 4, 1, DIR, REGS>(line, v, t0, lane);
+        stockham_pass<256, 4, 4, DIR, false>(line, v, t1, lane);
+        stockham_pass<256, 4, 16, DIR, false>(line, v, t2, lane);
+        stockham_pass<256, 4, 64, DIR, false>(line, v, t3, lane);
     }
 };
 template <>
 struct FftPlan<512> {
-    template <int DIR, bool REGS>
-    __device__ static void run(float2* line, const float2 (&v)[8], const float2* tw, int lane)
+    PassTw<512, 8, 1> t0;
+    PassTw<512, 8, 8> t1;
+    PassTw<512, 8, 64> t2;
+    __device__ void load(const float2* tw, int lane)
     {
-        stockham_pass<512, 8, 1, DIR, REGS>(line, v, tw, lane);
-        stockham_pass<512, 8, 8, DIR, false>(line, v, tw, lane);
-        stockham_pass<512, 8, 64, DIR, false>(line, v, tw, lane);
+        t0.load(tw, lane); t1.load(tw, lane); t2.load(tw, lane);
+    }
+    template <int DIR, bool REGS>
+    __device__ void run(float2* line, const float2 (&v)[8], int lane) const
+    {
+        stockham_pass<512, 8, 1, DIR, REGS>(line, v, t0, lane);
+        stockham_pass<512, 8, 8, DIR, false>(line, v, t1, lane);
+        stockham_pass<512, 8, 64, DIR, false>(line, v, t2, lane);
     }
 };
 template <>
 struct FftPlan<1024> {
-    template <int DIR, bool REGS>
-    __device__ static void run(float2* line, const float2 (&v)[16], const float2* tw, int lane)
+    PassTw<1024, 16, 1> t0;
+    PassTw<1024, 8, 16> t1;
+    PassTw<1024, 8, 128> t2;
+    __device__ void load(const float2* tw, int lane)
     {
-        stockham_pass<1024, 16, 1, DIR, REGS>(line, v, tw, lane);
-        stockham_pass<1024, 8, 16, DIR, false>(line, v, tw, lane);
-        stockham_pass<1024, 8, 128, DIR, false>(line, v, tw, lane);
+        t0.load(tw, lane); t1.load(tw, lane); t2.load(tw, lane);
+    }
+    template <int DIR, bool REGS>
+    __device__ void run(float2* line, const float2 (&v)[16], int lane) const
+    {
+        stockham_pass<1024, 16, 1, DIR, REGS>(line, v, t0, lane);
+        stockham_pass<1024, 8, 16, DIR, false>(line, v, t1, lane);
+        stockham_pass<1024, 8, 128, DIR, false>(line, v, t2, lane);
     }
 };
 template <>
 struct FftPlan<2048> {
-    template <int DIR, bool REGS>
-    __device__ static void run(float2* line, const float2 (&v)[32], const float2* tw, int lane)
+    PassTw<2048, 16, 1> t0;
+    PassTw<2048, 16, 16> t1;
+    PassTw<2048, 8, 256, false> t2;
+    __device__ void load(const float2* tw, int lane)
     {
-        stockham_pass<2048, 16, 1, DIR, REGS>(line, v, tw, lane);
-        stockham_pass<2048, 16, 16, DIR, false>(line, v, tw, lane);
-        stockham_pass<2048, 8, 256, DIR, false>(line, v, tw, lane);
+        t0.load(tw, lane); t1.load(tw, lane); t2.load(tw, lane);
+    }
+    template <int DIR, bool REGS>
+    __device__ void run(float2* line, const float2 (&v)[32], int lane) const
+    {
+        stockham_pass<2048, 16, 1, DIR, REGS>(line, v, t0, lane);
+        stockham_pass<2048, 16, 16, DIR, false>(line, v, t1, lane);
+        stockham_pass<2048, 8, 256, DIR, false>(line, v, t2, lane);
     }
 };
 
 // Transform a line held in registers (element lane + 64*i in v[i]); result in
-// `line` (LDS) in natural order.
-template <int M, int DIR>
-__device__ __forceinline__ void fft_from_regs(float2* line, const float2 (&v)[M / kWave],
-                                              const float2* tw, int lane)
+// `line` (LDS, padded slots) in natural order.  PL is FftPlan<M> or any plan
+// with the same interface (run<DIR, FROM_REGS>(line, v, lane)).
+template <int M, int DIR, class PL>
+__device__ __forceinline__ void fft_from_regs(const PL& P, float2* line,
+                                              const float2 (&v)[M / kWave], int lane)
 {
-    FftPlan<M>::template run<DIR, true>(line, v, tw, lane);
+    P.template run<DIR, true>(line, v, lane);
 }
 
 // Transform a line already in LDS, in place.
-template <int M, int DIR>
-__device__ __forceinline__ void fft_in_lds(float2* line, const float2* tw, int lane)
+template <int M, int DIR, class PL>
+__device__ __forceinline__ void fft_in_lds(const PL& P, float2* line, int lane)
 {
     float2 dummy[M / kWave];
-    FftPlan<M>::template run<DIR, false>(line, dummy, tw, lane);
+    P.template run<DIR, false>(line, dummy, lane);
 }
 
 }  // namespace srk
