@@ -74,6 +74,7 @@ def main():
     ap.add_argument("--kw", type=int, default=121)
     ap.add_argument("--m", type=int, default=0, help="tile/FFT size (default N)")
     ap.add_argument("--chunk", type=int, default=0, help="tiles per chunk (0 = auto)")
+    ap.add_argument("--p2-rows", type=int, default=0, help="k_y bands per row-pass block (0 = auto)")
     ap.add_argument("--scratch-mb", type=int, default=0)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--precision", default="both", choices=("fp32", "mixed_fp16", "both"))
@@ -121,8 +122,7 @@ def main():
     lf = torch.empty((B, N, N), device="cuda")
     lg = torch.empty((B, N, N), device="cuda")
     ws = srk.Workspace(N, W, e, M, B_max=B, scratch_mb=args.scratch_mb)
-    if args.chunk:
-        ws.set_chunk(args.chunk)
+    ws.set_tuning(chunk_tiles=args.chunk, p2_rows=args.p2_rows)
     stream = torch.cuda.current_stream()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 

@@ -98,13 +98,19 @@ int srkKernelSpectraSRSRK(int kw, double sigma, double dx, double cw, int M, voi
  * (training/srk_training_forward.cuh:117-158).  One workspace serves every
  * entry point for images of size N with the given erosion, batches up to
  * B_max, FFT/tile size M (the size of the spectra passed to the calls; M = N
- * for srkTrainingForward).  scratch_mb bounds the exchange buffers (0 = 256).
+ * for srkTrainingForward).  scratch_mb bounds the exchange buffers (0 = 1024).
  */
 int srkWorkspaceCreate(int N, int W, int erosion, int M, int B_max, size_t scratch_mb,
                        srk_workspace_t* ws);
 int srkWorkspaceDestroy(srk_workspace_t ws);   /* replaces SrkTrainWorkspace::release */
 size_t srkWorkspaceBytes(srk_workspace_t ws);  /* device bytes held */
 int srkWorkspaceSetChunk(srk_workspace_t ws, int tiles_per_chunk); /* 0 = automatic */
+/* tuning knobs (set by the autotuner; 0 restores the default) */
+enum {
+    SRK_TUNE_CHUNK_TILES = 0,  /* tiles per launch chunk (exchange buffers bound it)   */
+    SRK_TUNE_P2_ROWS = 1       /* k_y bands per row-pass block: 16, 8 or 4 (M <= 512) */
+};
+int srkWorkspaceSetTuning(srk_workspace_t ws, int key, int value);
 /* SRK_PREC_FP32 (default) or SRK_PREC_MIXED_FP16 for every later call on ws */
 int srkWorkspaceSetPrecision(srk_workspace_t ws, int precision);
 int srkWorkspaceInfo(srk_workspace_t ws, int* M, int* fwd_tiles_per_image,

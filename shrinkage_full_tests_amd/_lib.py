@@ -28,6 +28,7 @@ SIGNATURES = {
     "srkWorkspaceBytes": (_sz, [_p]),
     "srkWorkspaceSetChunk": (_i, [_p, _i]),
     "srkWorkspaceSetPrecision": (_i, [_p, _i]),
+    "srkWorkspaceSetTuning": (_i, [_p, _i, _i]),
     "srkWorkspaceInfo": (_i, [_p] + [ctypes.POINTER(_i)] * 5),
     "srkTrainingForward": (_i, [_p, _p, _p, _p, _i, _i, _i, _f, _i, _p, _p, _p]),
     "srkForwardOaS": (_i, [_p, _p, _p, _p, _i, _i, _i, _f, _p, _p, _p, _p, _p]),

@@ -211,6 +211,7 @@ bool adj_tiles(int N, int e, int M, std::vector<srk::Tile>& t, int& S2)
 struct srk_workspace_s {
     int N = 0, W = 0, e = 0, M = 0, B_max = 0;
     int chunk_override = 0;
+    int p2_rows = 0;
     int precision = SRK_PREC_FP32;
     srk::Tile* d_ftiles = nullptr;
     srk::Tile* d_atiles = nullptr;
@@ -261,6 +262,7 @@ srk::EngineArgs base_args(const srk_workspace_s* w, float dx)
     a.tw = w->tw;
     a.X1 = w->X1;
     a.X2 = w->X2;
+    a.p2_rows = w->p2_rows;
     return a;
 }
 
@@ -428,7 +430,7 @@ int srkWorkspaceCreate(int N, int W, int erosion, int M, int B_max, size_t scrat
     const int HP = srk::spec_pitch(M);
     const size_t x1_per = (size_t)4 * M * HP * sizeof(float2);
     const size_t x2_per = (size_t)std::max(4 * S2f, 5 * S2a) * HP * sizeof(float2);
-    const size_t budget = (scratch_mb ? scratch_mb : 256) << 20;
+    const size_t budget = (scratch_mb ? scratch_mb : 1024) << 20;
     size_t cmax = std::max<size_t>(1, budget / (x1_per + x2_per));
     cmax = std::min<size_t>(cmax, (size_t)B_max * std::max(w->nft, w->nat));
     w->x1_bytes = cmax * x1_per;
@@ -481,6 +483,7 @@ int srkWorkspaceCreate(int N, int W, int erosion, int M, int B_max, size_t scrat
         return st;
     }
     if (const char* env = std::getenv("SRK_CHUNK_TILES")) w->chunk_override = std::atoi(env);
+    if (const char* env = std::getenv("SRK_P2_ROWS")) w->p2_rows = std::atoi(env);
     *out = w;
     return SRK_OK;
 }
@@ -500,6 +503,23 @@ int srkWorkspaceSetChunk(srk_workspace_t w, int tiles)
     if (tiles < 0) return SRK_ERR_INVALID_ARG;
     w->chunk_override = tiles;
     return SRK_OK;
+}
+
+int srkWorkspaceSetTuning(srk_workspace_t w, int key, int value)
+{
+    if (!w) return SRK_ERR_NULL_PTR;
+    switch (key) {
+    case SRK_TUNE_CHUNK_TILES:
+        if (value < 0) return SRK_ERR_INVALID_ARG;
+        w->chunk_override = value;
+        return SRK_OK;
+    case SRK_TUNE_P2_ROWS:
+        if (value != 0 && value != 4 && value != 8 && value != 16) return SRK_ERR_INVALID_ARG;
+        w->p2_rows = value;
+        return SRK_OK;
+    default:
+        return SRK_ERR_INVALID_ARG;
+    }
 }
 
 int srkWorkspaceSetPrecision(srk_workspace_t w, int precision)

@@ -112,15 +112,25 @@ __global__ __launch_bounds__(256) void k_p1_cols(EngineArgs a, int nf)
             continue;
         }
         const float* src = (fi == 0 ? a.g0 : a.g1) + (long long)t.b * a.img_stride;
-        const float* colA = src + (long long)ca * N;
-        const float* colB = src + (long long)cb * N;
+        // clamped addresses keep every load in bounds; masks zero the padding
+        const float* colA = src + (long long)min(max(ca, 0), N - 1) * N;
+        const float* colB = src + (long long)min(max(cb, 0), N - 1) * N;
         float2 v[V];
+        if (va && vb && t.p0y >= 0 && t.p0y + M <= N) {
 #pragma unroll
-        for (int i = 0; i < V; ++i) {
-            const int row = t.p0y + lane + kWave * i;
-            const bool rv = row >= 0 && row < N;
-            v[i].x = (va && rv) ? colA[row] : 0.f;
-            v[i].y = (vb && rv) ? colB[row] : 0.f;
+            for (int i = 0; i < V; ++i) {
+                const int row = t.p0y + lane + kWave * i;
+                v[i] = make_float2(colA[row], colB[row]);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < V; ++i) {
+                const int row = t.p0y + lane + kWave * i;
+                const bool rv = row >= 0 && row < N;
+                const int rc = min(max(row, 0), N - 1);
+                const float xa = colA[rc], xb = colB[rc];
+                v[i] = make_float2((va && rv) ? xa : 0.f, (vb && rv) ? xb : 0.f);
+            }
         }
         fft_from_regs<M, -1>(FP, line, v, lane);
         unpack_store<M>(line, outA, outB, lane);
@@ -535,9 +545,9 @@ __global__ void k_import_spectra(const float2* __restrict__ Gx, const float2* __
 template <int M>
 struct P2Cfg;
 template <>
-struct P2Cfg<256> { static constexpr int RY = 16, NW = 16; };
+struct P2Cfg<256> { static constexpr int RY = 8, NW = 8; };
 template <>
-struct P2Cfg<512> { static constexpr int RY = 16, NW = 16; };
+struct P2Cfg<512> { static constexpr int RY = 8, NW = 8; };
 template <>
 struct P2Cfg<1024> { static constexpr int RY = 8, NW = 8; };
 template <>
@@ -551,42 +561,41 @@ struct IsMfmaPlan<FftPlanMfma<M>> { static constexpr bool value = true; };
 // Wavefronts per P2 block: one LDS line per wavefront, except that the MFMA
 // plan holds more per-lane constants and runs two lines per wavefront at
 // M >= 512 so that a lane may use 256 registers.
-template <int M, class PL>
-struct P2Sel {
-    static constexpr int RY = P2Cfg<M>::RY;
-    static constexpr int NW = (IsMfmaPlan<PL>::value && M >= 512) ? P2Cfg<M>::NW / 2 : P2Cfg<M>::NW;
+template <int M, int RY, class PL>
+struct P2Waves {
+    static constexpr int NW = (IsMfmaPlan<PL>::value && M >= 512 && RY >= 8) ? RY / 2 : RY;
 };
 
-template <int M>
+template <int M, int RY>
 constexpr size_t p2_lds()
 {
-    return 2ull * P2Cfg<M>::RY * (line_slots(M) + 1) * sizeof(float2);
+    return 2ull * RY * (line_slots(M) + 1) * sizeof(float2);
 }
 
-template <int M, class PL>
+template <int M, int RY, class PL>
 hipError_t prepare_kernels()
 {
     static bool done = false;
     static hipError_t st = hipSuccess;
     if (done) return st;
-    constexpr int RY = P2Sel<M, PL>::RY, NW = P2Sel<M, PL>::NW;
+    constexpr int NW = P2Waves<M, RY, PL>::NW;
     st = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_p2_conv<M, RY, NW, PL>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)p2_lds<M>());
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)p2_lds<M, RY>());
     if (st == hipSuccess)
         st = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_p2_corr<M, RY, NW, PL>),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)p2_lds<M>());
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)p2_lds<M, RY>());
     done = true;
     return st;
 }
 
-template <int M, class PL>
+template <int M, class PL, int RY>
 int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s)
 {
     constexpr int H = M / 2 + 1;
-    constexpr int RY = P2Sel<M, PL>::RY, NW = P2Sel<M, PL>::NW;
-    hipError_t st = prepare_kernels<M, PL>();
+    constexpr int NW = P2Waves<M, RY, PL>::NW;
+    hipError_t st = prepare_kernels<M, RY, PL>();
     if (st != hipSuccess) return (int)st;
-    const size_t lds2 = p2_lds<M>();
+    const size_t lds2 = p2_lds<M, RY>();
     const int cols_out = a0.S2 - 2 * a0.halo2;  // max output columns per tile
     for (int t0 = 0; t0 < ntiles; t0 += chunk) {
         const int c = std::min(chunk, ntiles - t0);
@@ -632,52 +641,43 @@ int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s)
     return 0;
 }
 
+// P2 band height: the tuned value if it is one of the compiled variants,
+// otherwise the default for M.
+template <int M, class PL>
+int run_rows(Op op, const EngineArgs& a, int ntiles, int chunk, int rows, hipStream_t s)
+{
+    if constexpr (M <= 512) {
+        if (rows == 16) return run_impl<M, PL, 16>(op, a, ntiles, chunk, s);
+        if (rows == 4) return run_impl<M, PL, 4>(op, a, ntiles, chunk, s);
+    }
+    return run_impl<M, PL, P2Cfg<M>::RY>(op, a, ntiles, chunk, s);
+}
+
 }  // namespace
 
 int engine_run(Op op, int M, int prec, const EngineArgs& a, int ntiles, int chunk, hipStream_t s)
 {
     if (chunk < 1 || ntiles < 0) return (int)hipErrorInvalidValue;
     if (ntiles == 0) return 0;
+    const int rows = a.p2_rows;
     if (prec == 0) {
         switch (M) {
-        case 256: return run_impl<256, FftPlan<256>>(op, a, ntiles, chunk, s);
-        case 512: return run_impl<512, FftPlan<512>>(op, a, ntiles, chunk, s);
-        case 1024: return run_impl<1024, FftPlan<1024>>(op, a, ntiles, chunk, s);
-        case 2048: return run_impl<2048, FftPlan<2048>>(op, a, ntiles, chunk, s);
+        case 256: return run_rows<256, FftPlan<256>>(op, a, ntiles, chunk, rows, s);
+        case 512: return run_rows<512, FftPlan<512>>(op, a, ntiles, chunk, rows, s);
+        case 1024: return run_rows<1024, FftPlan<1024>>(op, a, ntiles, chunk, rows, s);
+        case 2048: return run_rows<2048, FftPlan<2048>>(op, a, ntiles, chunk, rows, s);
         default: return (int)hipErrorInvalidValue;
         }
     }
     if (prec == 1) {
         switch (M) {
-        case 256: return run_impl<256, FftPlanMfma<256>>(op, a, ntiles, chunk, s);
-        case 512: return run_impl<512, FftPlanMfma<512>>(op, a, ntiles, chunk, s);
-        case 1024: return run_impl<1024, FftPlanMfma<1024>>(op, a, ntiles, chunk, s);
+        case 256: return run_rows<256, FftPlanMfma<256>>(op, a, ntiles, chunk, rows, s);
+        case 512: return run_rows<512, FftPlanMfma<512>>(op, a, ntiles, chunk, rows, s);
+        case 1024: return run_rows<1024, FftPlanMfma<1024>>(op, a, ntiles, chunk, rows, s);
         default: return (int)hipErrorInvalidValue;
         }
     }
     return (int)hipErrorInvalidValue;
-}
-
-size_t engine_p2_lds_bytes(int M)
-{
-    switch (M) {
-    case 256: return p2_lds<256>();
-    case 512: return p2_lds<512>();
-    case 1024: return p2_lds<1024>();
-    case 2048: return p2_lds<2048>();
-    default: return 0;
-    }
-}
-
-int engine_rows_per_block(int M)
-{
-    switch (M) {
-    case 256: return P2Cfg<256>::RY;
-    case 512: return P2Cfg<512>::RY;
-    case 1024: return P2Cfg<1024>::RY;
-    case 2048: return P2Cfg<2048>::RY;
-    default: return 0;
-    }
 }
 
 int engine_import_spectra(int M, const float2* Gx_ref, const float2* Gy_ref, float scale,

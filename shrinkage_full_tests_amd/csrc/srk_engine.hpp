@@ -55,6 +55,7 @@ struct EngineArgs {
     int np1, np2;          // planes per tile in X1 / X2
     int S2;                // columns per plane in X2
     int halo2;             // extra stored columns on each side in X2 (0 fwd, 1 adj)
+    int p2_rows;           // k_y bands per P2 block (tuning: 16, 8, 4; 0 = default)
 };
 
 // Padded pitch (complex elements) of one column spectrum in X1/X2.
@@ -83,8 +84,5 @@ int engine_adjoint_f_from_u(const float* v, const float* ux, const float* uy, fl
                             int N, int W, int e, int B, long long img_stride,
                             long long w_stride, hipStream_t stream);
 
-// Bytes of LDS the P2 kernels use for a given M (for capability checks).
-size_t engine_p2_lds_bytes(int M);
-int engine_rows_per_block(int M);
 
 }  // namespace srk

@@ -160,6 +160,47 @@ struct PassTw {
     }
 };
 
+// Slot arithmetic of the padded layout.  For a multiple of eight C,
+// lslot(a + C) = lslot(a) + C + C/8, so every access of a pass is a per-lane
+// base slot plus a compile-time offset (immediate LDS offsets, no per-access
+// address arithmetic).
+__host__ __device__ constexpr int slot_off(int C) { return C + C / 8; }
+
+// Base slot and constant offset of output r of butterfly j in a pass of radix R
+// and span Ns (element index (j/Ns)*Ns*R + j%Ns + r*Ns).
+template <int R, int Ns>
+struct WriteSlots {
+    __device__ static __forceinline__ int base(int j)
+    {
+        const int e = (j / Ns) * Ns * R + (j & (Ns - 1));
+        if constexpr (Ns % 8 == 0) {
+            return lslot(e);
+        } else if constexpr (Ns == 1 && R % 8 == 0) {
+            return slot_off(R * j);
+        } else if constexpr (Ns == 1 && R == 4) {
+            return 4 * j + (j >> 1);         // (4j + r) >> 3 = j >> 1 for r < 4
+        } else if constexpr (Ns == 4 && R == 4) {
+            return e + 2 * (j / 4);          // e = 16 (j/4) + j%4
+        } else {
+            return e;                        // generic: offset() adds the padding
+        }
+    }
+    __device__ static __forceinline__ int offset(int r, int e_unused)
+    {
+        if constexpr (Ns % 8 == 0) {
+            return slot_off(r * Ns);
+        } else if constexpr (Ns == 1 && R % 8 == 0) {
+            return r + (r >> 3);
+        } else if constexpr (Ns == 1 && R == 4) {
+            return r;
+        } else if constexpr (Ns == 4 && R == 4) {
+            return 4 * r + (r >= 2 ? 1 : 0);
+        } else {
+            return lslot(e_unused + r * Ns) - e_unused;
+        }
+    }
+};
+
 // One Stockham pass of radix R with span Ns over a line of M values.
 // FROM_REGS: operands come from v[] (element lane + 64*i in v[i]) instead of LDS.
 template <int M, int R, int Ns, int DIR, bool FROM_REGS, bool CACHE>
@@ -170,19 +211,21 @@ __device__ __forceinline__ void stockham_pass(float2* __restrict__ line, const f
     constexpr int G = NB / kWave;   // butterflies per lane
     static_assert(NB % kWave == 0, "radix plan must give >= 64 butterflies per pass");
     float2 x[G][R];
+    if constexpr (FROM_REGS) {
 #pragma unroll
-    for (int g = 0; g < G; ++g)
+        for (int g = 0; g < G; ++g)
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            if constexpr (FROM_REGS)
-                x[g][r] = v[g + r * G];
-            else
-                x[g][r] = line[lslot(lane + kWave * g + r * NB)];
-        }
+            for (int r = 0; r < R; ++r) x[g][r] = v[g + r * G];
+    } else {
+        const float2* rb = line + lslot(lane);
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int r = 0; r < R; ++r) x[g][r] = rb[slot_off(kWave * g + r * NB)];
+    }
 #pragma unroll
     for (int g = 0; g < G; ++g) {
         const int j = lane + kWave * g;
-        const int k = j & (Ns - 1);
         if constexpr (Ns > 1) {
 #pragma unroll
             for (int r = 1; r < R; ++r) {
@@ -191,9 +234,10 @@ __device__ __forceinline__ void stockham_pass(float2* __restrict__ line, const f
             }
         }
         dft<R, DIR>(x[g]);
-        const int base = (j / Ns) * Ns * R + k;
+        const int e0 = (j / Ns) * Ns * R + (j & (Ns - 1));
+        float2* wb = line + WriteSlots<R, Ns>::base(j);
 #pragma unroll
-        for (int r = 0; r < R; ++r) line[lslot(base + r * Ns)] = x[g][r];
+        for (int r = 0; r < R; ++r) wb[WriteSlots<R, Ns>::offset(r, e0)] = x[g][r];
     }
 }
 
