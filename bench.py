@@ -21,6 +21,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def pmc_traffic(N, B, kw, M):
+    """HBM bytes per fwd+adj step from the committed rocprofv3 --pmc summary of
+    this workload (FETCH_SIZE and WRITE_SIZE, copy-calibrated), or None."""
+    path = os.path.join(PROFILES, f"hbm_traffic_fwd_adj_n{N}_b{B}_kw{kw}_m{M}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_step:"):
+            kv = dict(x.split("=") for x in line.split(":", 1)[1].split())
+            return int(float(kv["fetch"]) + float(kv["write"]))
+    return None
 
 
 def alg_bytes(N, W, B, e_eq_i):
@@ -164,6 +178,26 @@ def main():
         stop.record(stream)
         torch.cuda.synchronize()
         ms_ = start.elapsed_time(stop) / args.steps
+        # per-kernel launch durations: the same steps run eagerly with every
+        # launch bracketed by HIP events on the launch stream
+        ws.set_timing(True)
+        ws.kernel_times()
+        torch.cuda.synchronize()
+        start.record(stream)
+        for _ in range(args.steps):
+            step()
+        stop.record(stream)
+        torch.cuda.synchronize()
+        ms_eager = start.elapsed_time(stop) / args.steps
+        kt = ws.kernel_times()
+        ws.set_timing(False)
+        kernels = {}
+        for name, (tot, cnt, tiles) in kt.items():
+            if cnt:
+                kernels[name] = {"ms_per_step": round(tot / args.steps, 3),
+                                 "launches_per_step": cnt // args.steps,
+                                 "avg_launch_us": round(1e3 * tot / cnt, 1),
+                                 "tiles_per_launch": round(tiles / cnt, 1)}
         # forward / adjoint split (eager, same stream)
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
         reps = max(1, min(3, args.steps))
@@ -178,19 +212,21 @@ def main():
             tf += ev[0].elapsed_time(ev[1]) / reps
             ta += ev[1].elapsed_time(ev[2]) / reps
         del graph
-        return ms_, tf, ta
+        return ms_, tf, ta, ms_eager, kernels
 
     mixed = None
     if args.precision in ("mixed_fp16", "both") and M <= 1024:
-        mms, mtf, mta = measure("mixed_fp16")
+        mms, mtf, mta, _, mk = measure("mixed_fp16")
         mixed = {"value": round(B * N * N / (mms / 1e3) / 1e6, 1), "unit": "Mpix/s",
                  "ms_per_step": round(mms, 3), "ms_fwd": round(mtf, 3), "ms_adj": round(mta, 3),
                  "dtype": "f16 MFMA operands, f32 accumulate",
-                 "tolerance": "tol_rel 1e-2, tol_abs 1e-3 vs FP64 (tests/test_gpu_mixed.py)"}
+                 "tolerance": "tol_rel 1e-2, tol_abs 1e-3 vs FP64 (tests/test_gpu_mixed.py)",
+                 "kernels": mk}
     if args.precision == "mixed_fp16" and mixed is not None:
-        ms, t_f, t_a = mms, mtf, mta
+        ms, t_f, t_a, ms_eager, kernels = mms, mtf, mta, None, mk
     else:
-        ms, t_f, t_a = measure("fp32")
+        ms, t_f, t_a, ms_eager, kernels = measure("fp32")
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"]) if kernels else None
     if mixed is not None:
         mixed["speedup_vs_fp32"] = round(ms / mixed["ms_per_step"], 3)
     graph = not args.no_graph
@@ -199,6 +235,7 @@ def main():
     nbytes = alg_bytes(N, W, B, True)
     achieved = nbytes / (ms / 1e3) / 1e9
     info = ws.info()
+    traffic = pmc_traffic(N, B, kw, M)
 
     secondary = None
     if not args.no_large:
@@ -268,9 +305,14 @@ def main():
         "ms_adj": round(t_a, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None,
-                     "scope": "one fwd+adj step (all passes), algorithmic bytes "
-                              f"16N^2+8W^2 per image = {nbytes / 1e9:.2f} GB"},
+                     "traffic": traffic,
+                     "scope": "one fwd+adj step over the batch (one hipGraph launch of all "
+                              "passes); algorithmic bytes 16N^2+8W^2 per image = "
+                              f"{nbytes / 1e9:.2f} GB per step; traffic = HBM bytes per step "
+                              "from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary in profiles/",
+                     "dominant_kernel": dom},
+        "ms_per_step_eager_timed": round(ms_eager, 3) if ms_eager else None,
+        "kernels": kernels,
         "mixed_fp16": mixed,
         "secondary": secondary,
         "cpu_baseline": cpu,

@@ -116,6 +116,21 @@ int srkWorkspaceSetPrecision(srk_workspace_t ws, int precision);
 int srkWorkspaceInfo(srk_workspace_t ws, int* M, int* fwd_tiles_per_image,
                      int* adj_tiles_per_image, int* chunk_fwd, int* chunk_adj);
 
+/*
+ * Per-launch timing of the engine kernels (instrumentation; no counterpart in
+ * the reference).  With timing enabled, every kernel launched by a later call
+ * on ws is bracketed by HIP events on that call's stream (skipped while the
+ * stream is being captured into a graph).  srkWorkspaceKernelTimes waits for
+ * the recorded launches and writes, per kernel kind k < SRK_KERNEL_KINDS,
+ * the summed duration in ms, the number of launches and the tiles they
+ * processed (any array may be null), then forgets the records.
+ */
+#define SRK_KERNEL_KINDS 6
+int srkWorkspaceSetTiming(srk_workspace_t ws, int enable);
+int srkWorkspaceKernelTimes(srk_workspace_t ws, double* total_ms, int* launches,
+                            long long* tiles);
+const char* srkKernelKindName(int kind);  /* "k_p1_cols", ..., or "" */
+
 int srkSynchronize(hipStream_t stream);
 
 /* ---- drop-in entry points ---------------------------------------------- */

@@ -22,6 +22,7 @@ __all__ = [
 ]
 
 SUPPORTED_M = (256, 512, 1024, 2048)
+KERNEL_KINDS = 6  # SRK_KERNEL_KINDS
 
 
 def _torch():
@@ -97,6 +98,20 @@ class Workspace:
         code = {"fp32": 0, "mixed_fp16": 1}.get(precision, precision)
         call("srkWorkspaceSetPrecision", self.handle, int(code))
         self.precision = precision
+
+    def set_timing(self, enable: bool = True) -> None:
+        """Bracket every later engine launch on this workspace with HIP events."""
+        call("srkWorkspaceSetTiming", self.handle, int(bool(enable)))
+
+    def kernel_times(self) -> dict:
+        """Wait for the timed launches; {kernel: (total ms, launches, tiles)}, then reset."""
+        n = KERNEL_KINDS
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_int * n)()
+        tiles = (ctypes.c_longlong * n)()
+        call("srkWorkspaceKernelTimes", self.handle, ctypes.addressof(ms), ctypes.addressof(cnt),
+             ctypes.addressof(tiles))
+        return {lib().srkKernelKindName(k).decode(): (ms[k], cnt[k], tiles[k]) for k in range(n)}
 
     def info(self) -> dict:
         v = [ctypes.c_int() for _ in range(5)]

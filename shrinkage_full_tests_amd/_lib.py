@@ -30,6 +30,9 @@ SIGNATURES = {
     "srkWorkspaceSetPrecision": (_i, [_p, _i]),
     "srkWorkspaceSetTuning": (_i, [_p, _i, _i]),
     "srkWorkspaceInfo": (_i, [_p] + [ctypes.POINTER(_i)] * 5),
+    "srkWorkspaceSetTiming": (_i, [_p, _i]),
+    "srkWorkspaceKernelTimes": (_i, [_p, _p, _p, _p]),
+    "srkKernelKindName": (ctypes.c_char_p, [_i]),
     "srkTrainingForward": (_i, [_p, _p, _p, _p, _i, _i, _i, _f, _i, _p, _p, _p]),
     "srkForwardOaS": (_i, [_p, _p, _p, _p, _i, _i, _i, _f, _p, _p, _p, _p, _p]),
     "adjointSRK": (_i, [_p, _p, _p, _p, _p, _p, _i, _i, _i, _f, _p, _p, _p, _p]),

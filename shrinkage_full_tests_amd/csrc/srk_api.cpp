@@ -208,11 +208,53 @@ bool adj_tiles(int N, int e, int M, std::vector<srk::Tile>& t, int& S2)
 
 }  // namespace
 
+static_assert(SRK_KERNEL_KINDS == srk::kKernelKinds, "kernel kinds of the C ABI and the engine");
+
+namespace srk {
+
+hipError_t LaunchTimer::record(hipStream_t s, hipEvent_t* ev)
+{
+    if (used == pool.size()) {
+        hipEvent_t e = nullptr;
+        const hipError_t st = hipEventCreate(&e);
+        if (st != hipSuccess) return st;
+        pool.push_back(e);
+    }
+    *ev = pool[used++];
+    return hipEventRecord(*ev, s);
+}
+
+hipError_t LaunchTimer::collect(double* ms, int* launches, long long* tiles)
+{
+    hipError_t st = hipSuccess;
+    for (const Rec& r : recs) {
+        float t = 0.f;
+        if (st == hipSuccess) st = hipEventSynchronize(r.e1);
+        if (st == hipSuccess) st = hipEventElapsedTime(&t, r.e0, r.e1);
+        if (st != hipSuccess || r.kind < 0 || r.kind >= kKernelKinds) continue;
+        if (ms) ms[r.kind] += t;
+        if (launches) launches[r.kind] += 1;
+        if (tiles) tiles[r.kind] += r.tiles;
+    }
+    recs.clear();
+    used = 0;
+    return st;
+}
+
+LaunchTimer::~LaunchTimer()
+{
+    for (hipEvent_t e : pool) hipEventDestroy(e);
+}
+
+}  // namespace srk
+
 struct srk_workspace_s {
     int N = 0, W = 0, e = 0, M = 0, B_max = 0;
     int chunk_override = 0;
     int p2_rows = 0;
     int precision = SRK_PREC_FP32;
+    bool timing = false;
+    srk::LaunchTimer timer;
     srk::Tile* d_ftiles = nullptr;
     srk::Tile* d_atiles = nullptr;
     int nft = 0, nat = 0, S2f = 0, S2a = 0;
@@ -302,6 +344,17 @@ int import_spectra(srk_workspace_s* w, const void* gx, const void* gy, float dx,
                                                  scale, w->GT, s));
 }
 
+// The launch timer to use on stream s: none unless timing is on and s is not
+// being captured (event records would become graph nodes).
+srk::LaunchTimer* timer_for(srk_workspace_s* w, hipStream_t s)
+{
+    if (!w->timing) return nullptr;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+        return nullptr;
+    return &w->timer;
+}
+
 int run_fwd(srk_workspace_s* w, srk::Op op, srk::EngineArgs a, int B, hipStream_t s)
 {
     const int nf = op == srk::Op::Tangent ? 2 : 1;
@@ -312,7 +365,8 @@ int run_fwd(srk_workspace_s* w, srk::Op op, srk::EngineArgs a, int B, hipStream_
     a.halo2 = 0;
     const int ntiles = B * w->nft;
     const int chunk = chunk_for(w, a.np1, a.np2, a.S2, ntiles);
-    return hip_status(srk::engine_run(op, w->M, w->precision, a, ntiles, chunk, s));
+    return hip_status(
+        srk::engine_run(op, w->M, w->precision, a, ntiles, chunk, s, timer_for(w, s)));
 }
 
 int run_adj(srk_workspace_s* w, srk::Op op, srk::EngineArgs a, int B, hipStream_t s)
@@ -328,7 +382,8 @@ int run_adj(srk_workspace_s* w, srk::Op op, srk::EngineArgs a, int B, hipStream_
     a.halo2 = 1;
     const int ntiles = B * w->nat;
     const int chunk = chunk_for(w, a.np1, a.np2, a.S2, ntiles);
-    return hip_status(srk::engine_run(op, w->M, w->precision, a, ntiles, chunk, s));
+    return hip_status(
+        srk::engine_run(op, w->M, w->precision, a, ntiles, chunk, s, timer_for(w, s)));
 }
 
 }  // namespace
@@ -520,6 +575,31 @@ int srkWorkspaceSetTuning(srk_workspace_t w, int key, int value)
     default:
         return SRK_ERR_INVALID_ARG;
     }
+}
+
+int srkWorkspaceSetTiming(srk_workspace_t w, int enable)
+{
+    if (!w) return SRK_ERR_NULL_PTR;
+    w->timing = enable != 0;
+    return SRK_OK;
+}
+
+int srkWorkspaceKernelTimes(srk_workspace_t w, double* total_ms, int* launches, long long* tiles)
+{
+    if (!w) return SRK_ERR_NULL_PTR;
+    for (int k = 0; k < SRK_KERNEL_KINDS; ++k) {
+        if (total_ms) total_ms[k] = 0.0;
+        if (launches) launches[k] = 0;
+        if (tiles) tiles[k] = 0;
+    }
+    return hip_status(w->timer.collect(total_ms, launches, tiles));
+}
+
+const char* srkKernelKindName(int kind)
+{
+    static const char* names[SRK_KERNEL_KINDS] = {"k_p1_cols", "k_p1_adj_src", "k_p2_conv",
+                                                  "k_p2_corr", "k_p3_fwd",    "k_p3_adj"};
+    return kind >= 0 && kind < SRK_KERNEL_KINDS ? names[kind] : "";
 }
 
 int srkWorkspaceSetPrecision(srk_workspace_t w, int precision)

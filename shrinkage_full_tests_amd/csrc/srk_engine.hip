@@ -589,7 +589,7 @@ hipError_t prepare_kernels()
 }
 
 template <int M, class PL, int RY>
-int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s)
+int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s, LaunchTimer* tm)
 {
     constexpr int H = M / 2 + 1;
     constexpr int NW = P2Waves<M, RY, PL>::NW;
@@ -604,37 +604,62 @@ int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s)
         const dim3 g1(M / 8, c), g1a(M / 4, c), g2((H + RY - 1) / RY, c);
         const dim3 g3f((cols_out + 3) / 4, c), g3a(((cols_out + 1) / 2 + 3) / 4, c);
         const dim3 b256(256), b2(NW * kWave);
+        // launch one pass, bracketed by timing events when a timer is given
+        auto run = [&](int kind, auto&& launch) {
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (tm && st == hipSuccess) st = tm->record(s, &e0);
+            launch();
+            if (tm && st == hipSuccess) {
+                st = tm->record(s, &e1);
+                tm->recs.push_back({kind, c, e0, e1});
+            }
+        };
+        auto p1 = [&](int nf) { run(kP1Cols, [&] { k_p1_cols<M, PL><<<g1, b256, 0, s>>>(a, nf); }); };
+        auto p1a = [&](int nf, int plane0) {
+            run(kP1AdjSrc, [&] { k_p1_adj_src<M, PL><<<g1a, b256, 0, s>>>(a, nf, plane0); });
+        };
+        auto p2c = [&](int nf) {
+            run(kP2Conv, [&] { k_p2_conv<M, RY, NW, PL><<<g2, b2, lds2, s>>>(a, nf); });
+        };
+        auto p2r = [&](int in, int out) {
+            run(kP2Corr, [&] { k_p2_corr<M, RY, NW, PL><<<g2, b2, lds2, s>>>(a, in, out); });
+        };
+        auto p3f = [&](int nf) { run(kP3Fwd, [&] { k_p3_fwd<M, PL><<<g3f, b256, 0, s>>>(a, nf); }); };
+        auto p3a = [&](int nf, int with_u, int planeL) {
+            run(kP3Adj, [&] { k_p3_adj<M, PL><<<g3a, b256, 0, s>>>(a, nf, with_u, planeL); });
+        };
         switch (op) {
         case Op::Forward:
-            k_p1_cols<M, PL><<<g1, b256, 0, s>>>(a, 1);
-            k_p2_conv<M, RY, NW, PL><<<g2, b2, lds2, s>>>(a, 1);
-            k_p3_fwd<M, PL><<<g3f, b256, 0, s>>>(a, 1);
+            p1(1);
+            p2c(1);
+            p3f(1);
             break;
         case Op::Tangent:
-            k_p1_cols<M, PL><<<g1, b256, 0, s>>>(a, 2);
-            k_p2_conv<M, RY, NW, PL><<<g2, b2, lds2, s>>>(a, 2);
-            k_p3_fwd<M, PL><<<g3f, b256, 0, s>>>(a, 2);
+            p1(2);
+            p2c(2);
+            p3f(2);
             break;
         case Op::Adjoint:
-            k_p1_cols<M, PL><<<g1, b256, 0, s>>>(a, 1);
-            k_p1_adj_src<M, PL><<<g1a, b256, 0, s>>>(a, 1, 1);
-            k_p2_conv<M, RY, NW, PL><<<g2, b2, lds2, s>>>(a, 1);
-            k_p2_corr<M, RY, NW, PL><<<g2, b2, lds2, s>>>(a, 1, 2);
-            k_p3_adj<M, PL><<<g3a, b256, 0, s>>>(a, 1, 1, 2);
+            p1(1);
+            p1a(1, 1);
+            p2c(1);
+            p2r(1, 2);
+            p3a(1, 1, 2);
             break;
         case Op::AdjointG:
-            k_p1_adj_src<M, PL><<<g1a, b256, 0, s>>>(a, 1, 0);
-            k_p2_corr<M, RY, NW, PL><<<g2, b2, lds2, s>>>(a, 0, 0);
-            k_p3_adj<M, PL><<<g3a, b256, 0, s>>>(a, 1, 0, 0);
+            p1a(1, 0);
+            p2r(0, 0);
+            p3a(1, 0, 0);
             break;
         case Op::TangentAdjoint:
-            k_p1_cols<M, PL><<<g1, b256, 0, s>>>(a, 2);
-            k_p1_adj_src<M, PL><<<g1a, b256, 0, s>>>(a, 2, 2);
-            k_p2_conv<M, RY, NW, PL><<<g2, b2, lds2, s>>>(a, 2);
-            k_p2_corr<M, RY, NW, PL><<<g2, b2, lds2, s>>>(a, 2, 4);
-            k_p3_adj<M, PL><<<g3a, b256, 0, s>>>(a, 2, 1, 4);
+            p1(2);
+            p1a(2, 2);
+            p2c(2);
+            p2r(2, 4);
+            p3a(2, 1, 4);
             break;
         }
+        if (st != hipSuccess) return (int)st;
         st = hipGetLastError();
         if (st != hipSuccess) return (int)st;
     }
@@ -644,36 +669,38 @@ int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s)
 // P2 band height: the tuned value if it is one of the compiled variants,
 // otherwise the default for M.
 template <int M, class PL>
-int run_rows(Op op, const EngineArgs& a, int ntiles, int chunk, int rows, hipStream_t s)
+int run_rows(Op op, const EngineArgs& a, int ntiles, int chunk, int rows, hipStream_t s,
+             LaunchTimer* tm)
 {
     if constexpr (M <= 512) {
-        if (rows == 16) return run_impl<M, PL, 16>(op, a, ntiles, chunk, s);
-        if (rows == 4) return run_impl<M, PL, 4>(op, a, ntiles, chunk, s);
+        if (rows == 16) return run_impl<M, PL, 16>(op, a, ntiles, chunk, s, tm);
+        if (rows == 4) return run_impl<M, PL, 4>(op, a, ntiles, chunk, s, tm);
     }
-    return run_impl<M, PL, P2Cfg<M>::RY>(op, a, ntiles, chunk, s);
+    return run_impl<M, PL, P2Cfg<M>::RY>(op, a, ntiles, chunk, s, tm);
 }
 
 }  // namespace
 
-int engine_run(Op op, int M, int prec, const EngineArgs& a, int ntiles, int chunk, hipStream_t s)
+int engine_run(Op op, int M, int prec, const EngineArgs& a, int ntiles, int chunk, hipStream_t s,
+               LaunchTimer* tm)
 {
     if (chunk < 1 || ntiles < 0) return (int)hipErrorInvalidValue;
     if (ntiles == 0) return 0;
     const int rows = a.p2_rows;
     if (prec == 0) {
         switch (M) {
-        case 256: return run_rows<256, FftPlan<256>>(op, a, ntiles, chunk, rows, s);
-        case 512: return run_rows<512, FftPlan<512>>(op, a, ntiles, chunk, rows, s);
-        case 1024: return run_rows<1024, FftPlan<1024>>(op, a, ntiles, chunk, rows, s);
-        case 2048: return run_rows<2048, FftPlan<2048>>(op, a, ntiles, chunk, rows, s);
+        case 256: return run_rows<256, FftPlan<256>>(op, a, ntiles, chunk, rows, s, tm);
+        case 512: return run_rows<512, FftPlan<512>>(op, a, ntiles, chunk, rows, s, tm);
+        case 1024: return run_rows<1024, FftPlan<1024>>(op, a, ntiles, chunk, rows, s, tm);
+        case 2048: return run_rows<2048, FftPlan<2048>>(op, a, ntiles, chunk, rows, s, tm);
         default: return (int)hipErrorInvalidValue;
         }
     }
     if (prec == 1) {
         switch (M) {
-        case 256: return run_rows<256, FftPlanMfma<256>>(op, a, ntiles, chunk, rows, s);
-        case 512: return run_rows<512, FftPlanMfma<512>>(op, a, ntiles, chunk, rows, s);
-        case 1024: return run_rows<1024, FftPlanMfma<1024>>(op, a, ntiles, chunk, rows, s);
+        case 256: return run_rows<256, FftPlanMfma<256>>(op, a, ntiles, chunk, rows, s, tm);
+        case 512: return run_rows<512, FftPlanMfma<512>>(op, a, ntiles, chunk, rows, s, tm);
+        case 1024: return run_rows<1024, FftPlanMfma<1024>>(op, a, ntiles, chunk, rows, s, tm);
         default: return (int)hipErrorInvalidValue;
         }
     }

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 namespace srk {
 
@@ -69,11 +70,41 @@ enum class Op : int {
     TangentAdjoint = 4,  // (d lambda_f, d lambda_g)
 };
 
+// Kernels of the engine, for per-launch timing.
+enum KernelKind : int {
+    kP1Cols = 0,   // k_p1_cols
+    kP1AdjSrc,     // k_p1_adj_src
+    kP2Conv,       // k_p2_conv
+    kP2Corr,       // k_p2_corr
+    kP3Fwd,        // k_p3_fwd
+    kP3Adj,        // k_p3_adj
+    kKernelKinds
+};
+
+// Host-side launch timer: when handed to engine_run, every launch is
+// bracketed by two HIP events recorded on the launch stream.  Events are
+// pooled and reused after collect().
+struct LaunchTimer {
+    struct Rec {
+        int kind, tiles;
+        hipEvent_t e0, e1;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+
+    hipError_t record(hipStream_t s, hipEvent_t* ev);
+    // Waits for the recorded launches and adds their durations (ms), counts
+    // and tile counts into the per-kind arrays; then forgets them.
+    hipError_t collect(double* ms, int* launches, long long* tiles);
+    ~LaunchTimer();
+};
+
 // Launch the passes of `op` for tiles [0, ntiles) of the tile table
 // args.tiles, chunk tiles at a time.  prec 0: FP32 transforms; 1: FP16 MFMA
 // DFT stages with FP32 accumulation (M <= 1024).  Returns a hipError_t.
 int engine_run(Op op, int M, int prec, const EngineArgs& args, int ntiles, int chunk,
-               hipStream_t stream);
+               hipStream_t stream, LaunchTimer* timer = nullptr);
 
 // Transpose + scale reference-layout spectra (Ghat[k_x*(M/2+1)+k_y]) into GT.
 int engine_import_spectra(int M, const float2* Gx_ref, const float2* Gy_ref, float scale,

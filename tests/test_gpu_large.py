@@ -135,6 +135,44 @@ def test_hipgraph_replay_matches_eager():
         _ = (E * 2).sum().item()   # eager work between replays
 
 
+def test_launch_timing_counts_launches_and_leaves_results_unchanged():
+    torch = _torch()
+    N, kw, M, B = 512, 121, 512, 4
+    e = 60
+    W = N - 2 * e
+    gx, gy = srk.kernel_spectra_srsrk(kw, 20.0, M)
+    E = patterns.grf_torch(B, N, seed=19)
+    v = torch.randn((B, W, W), device="cuda")
+    ws = srk.Workspace(N, W, e, M, B_max=B)
+    ws.set_chunk(3)   # two chunks per call: 3 + 1 tiles
+    R0 = srk.forward_batched(E, E, gx, gy, e, 1.0, ws)
+    l0 = srk.adjoint_batched(v, E, E, gx, gy, e, 1.0, ws)
+    ws.set_timing(True)
+    R1 = srk.forward_batched(E, E, gx, gy, e, 1.0, ws)
+    l1 = srk.adjoint_batched(v, E, E, gx, gy, e, 1.0, ws)
+    kt = ws.kernel_times()
+    assert torch.equal(R0, R1) and torch.equal(l0[0], l1[0]) and torch.equal(l0[1], l1[1])
+    want = {"k_p1_cols": 4, "k_p1_adj_src": 2, "k_p2_conv": 4, "k_p2_corr": 2, "k_p3_fwd": 2,
+            "k_p3_adj": 2}
+    for name, n in want.items():
+        ms, cnt, tiles = kt[name]
+        assert cnt == n and tiles == n // 2 * B and ms > 0, (name, kt[name])
+    assert ws.kernel_times()["k_p2_conv"] == (0.0, 0, 0)   # records were consumed
+    # launches captured into a graph are not timed, and the graph replays as before
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    R2 = torch.empty_like(R0)
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            srk.forward_batched(E, E, gx, gy, e, 1.0, ws, R2)
+    torch.cuda.current_stream().wait_stream(s)
+    g.replay()
+    torch.cuda.synchronize()
+    assert torch.equal(R2, R0)
+    assert all(cnt == 0 for _, cnt, _ in ws.kernel_times().values())
+
+
 def test_autograd_function_matches_adjoint():
     torch = _torch()
     N, kw, M, B = 256, 31, 256, 2
