@@ -22,30 +22,34 @@ __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 __device__ __forceinline__ int wave_id() { return threadIdx.x / kWave; }
 
 // Unpack the transform Z of two real sequences (p + i q) into the half spectra
-// P(k) = (Z(k) + conj Z(M-k))/2 and Q(k) = (Z(k) - conj Z(M-k))/(2i), k <= M/2.
+// P(k) = (Z(k) + conj Z(M-k))/2 and Q(k) = (Z(k) - conj Z(M-k))/(2i), k <= M/2,
+// stored in X1 band layout at column xp of planeP and column xq of planeQ.
 template <int M>
-__device__ __forceinline__ void unpack_store(const float2* line, float2* P, float2* Q, int lane)
+__device__ __forceinline__ void unpack_store(const float2* line, float2* planeP, int xp,
+                                             float2* planeQ, int xq, int rs, int lane)
 {
     for (int k = lane; k <= M / 2; k += kWave) {
         const float2 z = line[lslot(k)];
         const float2 zm = line[lslot((M - k) & (M - 1))];
-        P[k] = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
-        Q[k] = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
+        planeP[x1_index<M>(k, xp, rs)] = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
+        planeQ[x1_index<M>(k, xq, rs)] = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
     }
 }
 
+// zero column spectrum (k <= M/2) at column x of an X1 plane
 template <int M>
-__device__ __forceinline__ void zero_store(float2* P, int lane)
+__device__ __forceinline__ void zero_store(float2* plane, int x, int rs, int lane)
 {
-    for (int k = lane; k <= M / 2; k += kWave) P[k] = make_float2(0.f, 0.f);
+    for (int k = lane; k <= M / 2; k += kWave) plane[x1_index<M>(k, x, rs)] = make_float2(0.f, 0.f);
 }
 
-// Inverse transform of two Hermitian half spectra P, Q (either may be null for
-// zero) packed as Z = P + iQ; afterwards line[y] = p(y) + i q(y).  The DC and
-// Nyquist bins are taken as real, as a real inverse transform does.
+// Inverse transform of two Hermitian half spectra P, Q (X2 columns; either may
+// be null for zero) packed as Z = P + iQ; afterwards out[i] = p(y) + i q(y) at
+// tile row y = lane + 64 i.  The DC and Nyquist bins are taken as real, as a
+// real inverse transform does.
 template <int M, class PL>
 __device__ __forceinline__ void ifft_pair(const PL& FP, float2* line, const float2* P,
-                                          const float2* Q, int lane)
+                                          const float2* Q, float2 (&out)[M / kWave], int lane)
 {
     constexpr int V = M / kWave;
     float2 v[V];
@@ -61,7 +65,7 @@ __device__ __forceinline__ void ifft_pair(const PL& FP, float2* line, const floa
         }
         v[i] = k <= M / 2 ? make_float2(p.x - q.y, p.y + q.x) : make_float2(p.x + q.y, q.x - p.y);
     }
-    fft_from_regs<M, +1>(FP, line, v, lane);
+    fft_regs_to_regs<M, +1>(FP, line, v, out, lane);
 }
 
 __device__ __forceinline__ float grad_x(const float* f, int N, int r, int c)
@@ -104,11 +108,10 @@ __global__ __launch_bounds__(256) void k_p1_cols(EngineArgs a, int nf)
     const bool va = ca >= 0 && ca < N, vb = cb >= 0 && cb < N;
     float2* line = lds[w];
     for (int fi = 0; fi < nf; ++fi) {
-        float2* outA = a.X1 + (((long long)slot * a.np1 + fi) * M + xa) * HP;
-        float2* outB = outA + HP;
+        float2* plane = a.X1 + ((long long)slot * a.np1 + fi) * M * HP;
         if (!va && !vb) {
-            zero_store<M>(outA, lane);
-            zero_store<M>(outB, lane);
+            zero_store<M>(plane, xa, a.ry_log2, lane);
+            zero_store<M>(plane, xa + 1, a.ry_log2, lane);
             continue;
         }
         const float* src = (fi == 0 ? a.g0 : a.g1) + (long long)t.b * a.img_stride;
@@ -133,7 +136,7 @@ __global__ __launch_bounds__(256) void k_p1_cols(EngineArgs a, int nf)
             }
         }
         fft_from_regs<M, -1>(FP, line, v, lane);
-        unpack_store<M>(line, outA, outB, lane);
+        unpack_store<M>(line, plane, xa, plane, xa + 1, a.ry_log2, lane);
     }
 }
 
@@ -153,11 +156,11 @@ __global__ __launch_bounds__(256) void k_p1_adj_src(EngineArgs a, int nf, int pl
     const int N = a.N, e = a.e, W = a.W;
     const int x = blockIdx.x * 4 + w;
     const int gx = t.p0x + x;
-    float2* outX = a.X1 + (((long long)slot * a.np1 + plane0) * M + x) * HP;
-    float2* outY = a.X1 + (((long long)slot * a.np1 + plane0 + 1) * M + x) * HP;
+    float2* planeX = a.X1 + ((long long)slot * a.np1 + plane0) * M * HP;
+    float2* planeY = planeX + (long long)M * HP;
     if (gx < e || gx >= N - e) {
-        zero_store<M>(outX, lane);
-        zero_store<M>(outY, lane);
+        zero_store<M>(planeX, x, a.ry_log2, lane);
+        zero_store<M>(planeY, x, a.ry_log2, lane);
         return;
     }
     const float* f0 = a.f0 + (long long)t.b * a.img_stride;
@@ -185,128 +188,191 @@ __global__ __launch_bounds__(256) void k_p1_adj_src(EngineArgs a, int nf, int pl
     PL FP;
     FP.load(a.tw, lane);
     fft_from_regs<M, -1>(FP, line, v, lane);
-    unpack_store<M>(line, outX, outY, lane);
+    unpack_store<M>(line, planeX, x, planeY, x, a.ry_log2, lane);
 }
 
-// Shared helpers of the P2 kernels: transposed load of RY k_y bands of one X1
-// plane into LDS lines (line r holds band k0 + r along x), and the transposed
-// store of LDS lines into an X2 plane for tile-local columns [lo, lo + cnt).
-template <int M, int RY, int LP>
-__device__ __forceinline__ void load_bands(float2* L, const float2* plane, int k0, int nrows)
+// ---------------------------------------------------------------------------
+// P2 geometry.  A block owns RY consecutive k_y rows (a band) of one tile.
+// The M/(2 RY) full bands come first in launch order, arranged so that block
+// L runs band L mod 8 (+ 8 j) on XCD L mod 8 (workgroups are dispatched to the
+// eight XCDs round robin): every XCD only ever reads its own eighth of the
+// spectra, which stays resident in its L2.  The last band (the Nyquist
+// row k_y = M/2 alone) follows, one block per tile.
+// ---------------------------------------------------------------------------
+template <int M, int RY>
+struct P2Geom {
+    static constexpr int NBF = M / (2 * RY);   // full bands per tile
+    static_assert(NBF % 8 == 0, "full bands must split evenly over the eight XCDs");
+    static constexpr int blocks(int tiles) { return (NBF + 1) * tiles; }
+    int slot, band;
+    __device__ P2Geom()
+    {
+        const int L = blockIdx.x;
+        const int tiles = gridDim.x / (NBF + 1);
+        if (L < NBF * tiles) {
+            constexpr int BPX = NBF / 8;   // bands per XCD
+            const int q = L >> 3;
+            band = (L & 7) + 8 * (q % BPX);
+            slot = q / BPX;
+        } else {
+            band = NBF;
+            slot = L - NBF * tiles;
+        }
+    }
+    __device__ int rows() const { return band < NBF ? RY : 1; }
+};
+
+// LDS pitch of the P2 band lines: line_slots(M) + 32/RY, so that the
+// transposing accesses below (lane -> (row lane % RY, column lane / RY)) fall
+// on distinct banks within each half wavefront.
+template <int M, int RY>
+constexpr int p2_pitch() { return line_slots(M) + 32 / RY; }
+
+// Band of one X1 plane (M x RY elements, contiguous) -> LDS lines: line r holds
+// k_y row k0 + r along x.  Rows >= nrows are zero.
+template <int M, int RY, int NW>
+__device__ __forceinline__ void load_band(float2* L, const float2* __restrict__ band, int nrows)
 {
-    constexpr int HP = spec_pitch(M);
-    for (int idx = threadIdx.x; idx < M * RY; idx += blockDim.x) {
-        const int x = idx / RY, r = idx - x * RY;
-        L[r * LP + lslot(x)] = r < nrows ? plane[(long long)x * HP + k0 + r] : make_float2(0.f, 0.f);
+    constexpr int T = NW * kWave, XS = T / RY, IT = M * RY / T, LP = p2_pitch<M, RY>();
+    static_assert(XS % 8 == 0 && (M * RY) % T == 0, "band load shape");
+    const int r = threadIdx.x % RY, x0 = threadIdx.x / RY;
+    float2* d = L + r * LP + lslot(x0);
+    const float2* src = band + threadIdx.x;
+    const bool ok = r < nrows;
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        float2 v = make_float2(0.f, 0.f);
+        if (ok) v = src[i * T];
+        d[i * slot_off(XS)] = v;
     }
 }
 
-template <int M, int RY, int LP>
-__device__ __forceinline__ void store_bands(float2* plane, const float2* L, int k0, int nrows,
-                                            int lo, int cnt)
+// LDS lines -> X2 plane ([x][k_y], pitch HP) for tile-local columns
+// [lo, lo + cnt); columns outside [0, M) are stored as zero.
+template <int M, int RY, int NW>
+__device__ __forceinline__ void store_band(float2* __restrict__ plane, const float2* L, int k0,
+                                           int nrows, int lo, int cnt)
 {
-    constexpr int HP = spec_pitch(M);
-    for (int idx = threadIdx.x; idx < cnt * RY; idx += blockDim.x) {
-        const int xi = idx / RY, r = idx - xi * RY;
-        if (r < nrows) {
-            const int x = lo + xi;
-            plane[(long long)xi * HP + k0 + r] =
-                (x >= 0 && x < M) ? L[r * LP + lslot(x)] : make_float2(0.f, 0.f);
-        }
+    constexpr int T = NW * kWave, XS = T / RY, HP = spec_pitch(M), LP = p2_pitch<M, RY>();
+    const int r = threadIdx.x % RY, xi0 = threadIdx.x / RY;
+    if (r >= nrows) return;
+    const float2* src = L + r * LP;
+    float2* d = plane + (long long)xi0 * HP + k0 + r;
+    for (int xi = xi0; xi < cnt; xi += XS, d += (long long)XS * HP) {
+        const int x = lo + xi;
+        float2 v = make_float2(0.f, 0.f);
+        if (x >= 0 && x < M) v = src[lslot(x)];
+        *d = v;
     }
 }
 
 // ---------------------------------------------------------------------------
-// P2 conv: x-FFT of each convolved-field band, products with Gx and Gy,
-// inverse x-FFTs, crop; field fi -> X2 planes 2fi (u_x) and 2fi+1 (u_y).
+// P2 conv: x-FFT of each convolved-field band row, products with Gx and Gy in
+// registers, inverse x-FFTs, crop; field fi -> X2 planes 2fi (u_x), 2fi+1 (u_y).
 // ---------------------------------------------------------------------------
 template <int M, int RY, int NW, class PL>
 __global__ __launch_bounds__(NW * kWave) void k_p2_conv(EngineArgs a, int nf)
 {
+    constexpr int V = M / kWave;
     constexpr int H = M / 2 + 1;
     constexpr int HP = spec_pitch(M);
-    constexpr int LP = line_slots(M) + 1;
+    constexpr int LP = p2_pitch<M, RY>();
     extern __shared__ float2 sm[];
     float2* A = sm;
     float2* Bs = sm + RY * LP;
     const int lane = lane_id(), w = wave_id();
     PL FP;
     FP.load(a.tw, lane);
-    const int slot = blockIdx.y;
+    const P2Geom<M, RY> geo;
+    const int slot = geo.slot;
     const Tile t = a.tiles[slot];
-    const int k0 = blockIdx.x * RY;
-    const int nrows = min(RY, H - k0);
+    const int k0 = geo.band * RY;
+    const int nrows = geo.rows();
     const int lo = t.ox0 - t.p0x - a.halo2;
     const int cnt = t.ox1 - t.ox0 + 2 * a.halo2;
     for (int fi = 0; fi < nf; ++fi) {
-        const float2* src = a.X1 + ((long long)slot * a.np1 + fi) * M * HP;
-        load_bands<M, RY, LP>(A, src, k0, nrows);
+        const float2* src = a.X1 + ((long long)slot * a.np1 + fi) * M * HP + (long long)k0 * M;
+        load_band<M, RY, NW>(A, src, nrows);
         __syncthreads();
         for (int r = w; r < nrows; r += NW) {
             float2* la = A + r * LP;
             float2* lb = Bs + r * LP;
-            fft_in_lds<M, -1>(FP, la, lane);
-            const float2* gxr = a.GT + (long long)(k0 + r) * M;
-            const float2* gyr = a.GT + (long long)(H + k0 + r) * M;
-            for (int kx = lane; kx < M; kx += kWave) {
-                const float2 X = la[lslot(kx)];
-                lb[lslot(kx)] = cmul(X, gyr[kx]);
-                la[lslot(kx)] = cmul(X, gxr[kx]);
+            const float2* gxr = a.GT + (long long)(k0 + r) * M + lane;
+            const float2* gyr = a.GT + (long long)(H + k0 + r) * M + lane;
+            float2 gxv[V], gyv[V];
+#pragma unroll
+            for (int i = 0; i < V; ++i) {
+                gxv[i] = gxr[kWave * i];
+                gyv[i] = gyr[kWave * i];
             }
-            fft_in_lds<M, +1>(FP, la, lane);
-            fft_in_lds<M, +1>(FP, lb, lane);
+            float2 X[V], U[V];
+            fft_lds_to_regs<M, -1>(FP, la, X, lane);
+#pragma unroll
+            for (int i = 0; i < V; ++i) {
+                U[i] = cmul(X[i], gyv[i]);
+                X[i] = cmul(X[i], gxv[i]);
+            }
+            fft_from_regs<M, +1>(FP, la, X, lane);
+            fft_from_regs<M, +1>(FP, lb, U, lane);
         }
         __syncthreads();
         float2* dstA = a.X2 + ((long long)slot * a.np2 + 2 * fi) * a.S2 * HP;
         float2* dstB = a.X2 + ((long long)slot * a.np2 + 2 * fi + 1) * a.S2 * HP;
-        store_bands<M, RY, LP>(dstA, A, k0, nrows, lo, cnt);
-        store_bands<M, RY, LP>(dstB, Bs, k0, nrows, lo, cnt);
+        store_band<M, RY, NW>(dstA, A, k0, nrows, lo, cnt);
+        store_band<M, RY, NW>(dstB, Bs, k0, nrows, lo, cnt);
         __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------
-// P2 corr: x-FFT of the adjoint sources, L = conj(Gx) A_x + conj(Gy) A_y,
-// inverse x-FFT, crop -> X2 plane planeOut.
+// P2 corr: x-FFT of the adjoint sources, L = conj(Gx) A_x + conj(Gy) A_y in
+// registers, inverse x-FFT, crop -> X2 plane planeOut.
 // ---------------------------------------------------------------------------
 template <int M, int RY, int NW, class PL>
 __global__ __launch_bounds__(NW * kWave) void k_p2_corr(EngineArgs a, int planeIn, int planeOut)
 {
+    constexpr int V = M / kWave;
     constexpr int H = M / 2 + 1;
     constexpr int HP = spec_pitch(M);
-    constexpr int LP = line_slots(M) + 1;
+    constexpr int LP = p2_pitch<M, RY>();
     extern __shared__ float2 sm[];
     float2* A = sm;
     float2* Bs = sm + RY * LP;
     const int lane = lane_id(), w = wave_id();
     PL FP;
     FP.load(a.tw, lane);
-    const int slot = blockIdx.y;
+    const P2Geom<M, RY> geo;
+    const int slot = geo.slot;
     const Tile t = a.tiles[slot];
-    const int k0 = blockIdx.x * RY;
-    const int nrows = min(RY, H - k0);
+    const int k0 = geo.band * RY;
+    const int nrows = geo.rows();
     const int lo = t.ox0 - t.p0x - a.halo2;
     const int cnt = t.ox1 - t.ox0 + 2 * a.halo2;
-    load_bands<M, RY, LP>(A, a.X1 + ((long long)slot * a.np1 + planeIn) * M * HP, k0, nrows);
-    load_bands<M, RY, LP>(Bs, a.X1 + ((long long)slot * a.np1 + planeIn + 1) * M * HP, k0, nrows);
+    const float2* src = a.X1 + ((long long)slot * a.np1 + planeIn) * M * HP + (long long)k0 * M;
+    load_band<M, RY, NW>(A, src, nrows);
+    load_band<M, RY, NW>(Bs, src + (long long)M * HP, nrows);
     __syncthreads();
     for (int r = w; r < nrows; r += NW) {
         float2* la = A + r * LP;
         float2* lb = Bs + r * LP;
-        fft_in_lds<M, -1>(FP, la, lane);
-        fft_in_lds<M, -1>(FP, lb, lane);
-        const float2* gxr = a.GT + (long long)(k0 + r) * M;
-        const float2* gyr = a.GT + (long long)(H + k0 + r) * M;
-        for (int kx = lane; kx < M; kx += kWave) {
-            const float2 p = cmulc(la[lslot(kx)], gxr[kx]);
-            const float2 q = cmulc(lb[lslot(kx)], gyr[kx]);
-            la[lslot(kx)] = cadd(p, q);
+        const float2* gxr = a.GT + (long long)(k0 + r) * M + lane;
+        const float2* gyr = a.GT + (long long)(H + k0 + r) * M + lane;
+        float2 gxv[V], gyv[V];
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            gxv[i] = gxr[kWave * i];
+            gyv[i] = gyr[kWave * i];
         }
-        fft_in_lds<M, +1>(FP, la, lane);
+        float2 X[V], Y[V];
+        fft_lds_to_regs<M, -1>(FP, la, X, lane);
+        fft_lds_to_regs<M, -1>(FP, lb, Y, lane);
+#pragma unroll
+        for (int i = 0; i < V; ++i) X[i] = cadd(cmulc(X[i], gxv[i]), cmulc(Y[i], gyv[i]));
+        fft_from_regs<M, +1>(FP, la, X, lane);
     }
     __syncthreads();
-    store_bands<M, RY, LP>(a.X2 + ((long long)slot * a.np2 + planeOut) * a.S2 * HP, A, k0, nrows,
-                           lo, cnt);
+    store_band<M, RY, NW>(a.X2 + ((long long)slot * a.np2 + planeOut) * a.S2 * HP, A, k0, nrows,
+                          lo, cnt);
 }
 
 // ---------------------------------------------------------------------------
@@ -334,27 +400,22 @@ __global__ __launch_bounds__(256) void k_p3_fwd(EngineArgs a, int nf)
     const float* f0 = a.f0 + (long long)t.b * a.img_stride;
     float* out = a.out0 + (long long)t.b * a.w_stride + (long long)(gx - e) * W;
 
-    // u(g) for this column
-    ifft_pair<M, PL>(FP, line, base, base + pstride, lane);
+    // u(g) for this column at tile rows y = lane + 64 j
     float2 u0[V];
-#pragma unroll
-    for (int j = 0; j < V; ++j) {
-        const int gy = t.oy0 + lane + kWave * j;
-        u0[j] = gy < t.oy1 ? line[lslot(gy - t.p0y)] : make_float2(0.f, 0.f);
-    }
+    ifft_pair<M, PL>(FP, line, base, base + pstride, u0, lane);
     if (nf == 1) {
 #pragma unroll
         for (int j = 0; j < V; ++j) {
-            const int gy = t.oy0 + lane + kWave * j;
-            if (gy < t.oy1)
+            const int gy = t.p0y + lane + kWave * j;
+            if (gy >= t.oy0 && gy < t.oy1)
                 out[gy - e] = u0[j].x * grad_x(f0, N, gy, gx) + u0[j].y * grad_y(f0, N, gy, gx);
         }
         if (a.ux_out) {
             const long long o = (long long)t.b * a.w_stride + (long long)(gx - e) * W - e;
 #pragma unroll
             for (int j = 0; j < V; ++j) {
-                const int gy = t.oy0 + lane + kWave * j;
-                if (gy < t.oy1) {
+                const int gy = t.p0y + lane + kWave * j;
+                if (gy >= t.oy0 && gy < t.oy1) {
                     a.ux_out[o + gy] = u0[j].x;
                     a.uy_out[o + gy] = u0[j].y;
                 }
@@ -364,13 +425,13 @@ __global__ __launch_bounds__(256) void k_p3_fwd(EngineArgs a, int nf)
     }
     // tangent: + u(dg) . grad f + u(g) . grad df
     const float* f1 = a.f1 + (long long)t.b * a.img_stride;
-    ifft_pair<M, PL>(FP, line, base + 2 * pstride, base + 3 * pstride, lane);
+    float2 u1[V];
+    ifft_pair<M, PL>(FP, line, base + 2 * pstride, base + 3 * pstride, u1, lane);
 #pragma unroll
     for (int j = 0; j < V; ++j) {
-        const int gy = t.oy0 + lane + kWave * j;
-        if (gy < t.oy1) {
-            const float2 u1 = line[lslot(gy - t.p0y)];
-            out[gy - e] = u1.x * grad_x(f0, N, gy, gx) + u1.y * grad_y(f0, N, gy, gx) +
+        const int gy = t.p0y + lane + kWave * j;
+        if (gy >= t.oy0 && gy < t.oy1) {
+            out[gy - e] = u1[j].x * grad_x(f0, N, gy, gx) + u1[j].y * grad_y(f0, N, gy, gx) +
                           u0[j].x * grad_x(f1, N, gy, gx) + u0[j].y * grad_y(f1, N, gy, gx);
         }
     }
@@ -434,29 +495,21 @@ __global__ __launch_bounds__(256) void k_p3_adj(EngineArgs a, int nf, int with_u
                 }
             }
             const int px = 2 * fi, py = 2 * fi + 1;
+            float2 u[V];
             // u_x at gx-1, gx+1 -> x-stencil transpose at gx
-            ifft_pair<M, PL>(FP, line, col(px, -1), col(px, 1), lane);
+            ifft_pair<M, PL>(FP, line, col(px, -1), col(px, 1), u, lane);
 #pragma unroll
-            for (int j = 0; j < V; ++j) {
-                const float2 u = line[lslot(lane + kWave * j)];
-                acc0[j] += 0.5f * (u.x * cv[0][j] - u.y * cv[2][j]);
-            }
+            for (int j = 0; j < V; ++j) acc0[j] += 0.5f * (u[j].x * cv[0][j] - u[j].y * cv[2][j]);
             // u_x at gx, gx+2 -> x-stencil transpose at gx+1
-            ifft_pair<M, PL>(FP, line, col(px, 0), col(px, 2), lane);
+            ifft_pair<M, PL>(FP, line, col(px, 0), col(px, 2), u, lane);
 #pragma unroll
-            for (int j = 0; j < V; ++j) {
-                const float2 u = line[lslot(lane + kWave * j)];
-                acc1[j] += 0.5f * (u.x * cv[1][j] - u.y * cv[3][j]);
-            }
-            // u_y at gx, gx+1: w_y = u_y v formed in place, then the y-stencil
+            for (int j = 0; j < V; ++j) acc1[j] += 0.5f * (u[j].x * cv[1][j] - u[j].y * cv[3][j]);
+            // u_y at gx, gx+1: w_y = u_y v written to the line, then the y-stencil
             // transpose reads the neighbouring rows (same wavefront, in order).
-            ifft_pair<M, PL>(FP, line, col(py, 0), col(py, 1), lane);
+            ifft_pair<M, PL>(FP, line, col(py, 0), col(py, 1), u, lane);
 #pragma unroll
-            for (int j = 0; j < V; ++j) {
-                const int y = lane + kWave * j;
-                const float2 u = line[lslot(y)];
-                line[lslot(y)] = make_float2(u.x * cv[1][j], u.y * cv[2][j]);
-            }
+            for (int j = 0; j < V; ++j)
+                line[lslot(lane + kWave * j)] = make_float2(u[j].x * cv[1][j], u[j].y * cv[2][j]);
 #pragma unroll
             for (int j = 0; j < V; ++j) {
                 const int y = lane + kWave * j;
@@ -477,15 +530,15 @@ __global__ __launch_bounds__(256) void k_p3_adj(EngineArgs a, int nf, int with_u
         }
     }
     // lambda_g
-    ifft_pair<M, PL>(FP, line, col(planeL, 0), col(planeL, 1), lane);
+    float2 l[V];
+    ifft_pair<M, PL>(FP, line, col(planeL, 0), col(planeL, 1), l, lane);
     float* o1 = a.out1 + ib + (long long)gx * N;
 #pragma unroll
     for (int j = 0; j < V; ++j) {
-        const int gy = t.oy0 + lane + kWave * j;
-        if (gy < t.oy1) {
-            const float2 l = line[lslot(gy - t.p0y)];
-            o1[gy] = l.x;
-            if (second) o1[N + gy] = l.y;
+        const int gy = t.p0y + lane + kWave * j;
+        if (gy >= t.oy0 && gy < t.oy1) {
+            o1[gy] = l[j].x;
+            if (second) o1[N + gy] = l[j].y;
         }
     }
 }
@@ -569,7 +622,7 @@ struct P2Waves {
 template <int M, int RY>
 constexpr size_t p2_lds()
 {
-    return 2ull * RY * (line_slots(M) + 1) * sizeof(float2);
+    return 2ull * RY * p2_pitch<M, RY>() * sizeof(float2);
 }
 
 template <int M, int RY, class PL>
@@ -591,7 +644,7 @@ hipError_t prepare_kernels()
 template <int M, class PL, int RY>
 int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s, LaunchTimer* tm)
 {
-    constexpr int H = M / 2 + 1;
+    static_assert(RY == 16 || RY == 8 || RY == 4, "band heights");
     constexpr int NW = P2Waves<M, RY, PL>::NW;
     hipError_t st = prepare_kernels<M, RY, PL>();
     if (st != hipSuccess) return (int)st;
@@ -601,7 +654,8 @@ int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s, 
         const int c = std::min(chunk, ntiles - t0);
         EngineArgs a = a0;
         a.tiles = a0.tiles + t0;
-        const dim3 g1(M / 8, c), g1a(M / 4, c), g2((H + RY - 1) / RY, c);
+        a.ry_log2 = RY == 16 ? 4 : RY == 8 ? 3 : 2;
+        const dim3 g1(M / 8, c), g1a(M / 4, c), g2(P2Geom<M, RY>::blocks(c));
         const dim3 g3f((cols_out + 3) / 4, c), g3a(((cols_out + 1) / 2 + 3) / 4, c);
         const dim3 b256(256), b2(NW * kWave);
         // launch one pass, bracketed by timing events when a timer is given

@@ -4,7 +4,8 @@
 // passes, with exchange buffers between them sized for a chunk of tiles:
 //
 //   P1  (columns)  real FFT along y of every tile column (contiguous loads),
-//                  written column-spectrum-major: X1[tile][plane][x][k_y];
+//                  written band-major: X1[tile][plane][k_y / RY][x][k_y % RY],
+//                  so that each P2 block reads one contiguous band;
 //   P2  (rows)     per k_y band: FFT along x, product with the kernel
 //                  spectra, inverse FFT along x, crop to the needed columns:
 //                  X2[tile][plane][x][k_y];
@@ -51,16 +52,27 @@ struct EngineArgs {
     const float2* GT;      // [2][M/2+1][M] scaled spectra (Gx then Gy)
     const float2* tw;      // twiddles exp(-2 pi i k / M), k < M
     const Tile* tiles;     // tiles of this chunk
-    float2* X1;            // [chunk][np1][M][HP]
+    float2* X1;            // [chunk][np1] planes of M * HP elements, band layout (x1_index)
     float2* X2;            // [chunk][np2][S2][HP]
     int np1, np2;          // planes per tile in X1 / X2
     int S2;                // columns per plane in X2
     int halo2;             // extra stored columns on each side in X2 (0 fwd, 1 adj)
     int p2_rows;           // k_y bands per P2 block (tuning: 16, 8, 4; 0 = default)
+    int ry_log2;           // log2 of the band height in effect (set by the engine)
 };
 
-// Padded pitch (complex elements) of one column spectrum in X1/X2.
-constexpr int spec_pitch(int M) { return M / 2 + 16; }
+// Padded pitch (complex elements) of one column spectrum in X2, and the
+// plane size M * spec_pitch(M) of both exchange buffers.
+constexpr __host__ __device__ int spec_pitch(int M) { return M / 2 + 16; }
+
+// X1 plane index of (k_y, x) in band layout with bands of 2^rs rows: bands of
+// M * 2^rs elements, x-major inside a band.  The last band (k_y = M/2 only)
+// ends at (M/2 + 2^rs) * M <= M * spec_pitch(M) for 2^rs <= 16.
+template <int M>
+__host__ __device__ constexpr int x1_index(int k, int x, int rs)
+{
+    return (((k >> rs) * M + x) << rs) + (k & ((1 << rs) - 1));
+}
 
 enum class Op : int {
     Forward = 0,         // h = u(g) . grad f

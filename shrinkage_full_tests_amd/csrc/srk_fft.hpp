@@ -202,14 +202,18 @@ struct WriteSlots {
 };
 
 // One Stockham pass of radix R with span Ns over a line of M values.
-// FROM_REGS: operands come from v[] (element lane + 64*i in v[i]) instead of LDS.
-template <int M, int R, int Ns, int DIR, bool FROM_REGS, bool CACHE>
+// FROM_REGS: operands come from v[] (element lane + 64*i in v[i]) instead of
+// LDS.  TO_REGS (last pass only, Ns * R = M): outputs stay in registers,
+// element lane + 64*i in out[i], instead of being written to LDS.
+template <int M, int R, int Ns, int DIR, bool FROM_REGS, bool TO_REGS, bool CACHE>
 __device__ __forceinline__ void stockham_pass(float2* __restrict__ line, const float2 (&v)[M / kWave],
+                                              float2 (&out)[M / kWave],
                                               const PassTw<M, R, Ns, CACHE>& T, int lane)
 {
     constexpr int NB = M / R;       // butterflies in the pass
     constexpr int G = NB / kWave;   // butterflies per lane
     static_assert(NB % kWave == 0, "radix plan must give >= 64 butterflies per pass");
+    static_assert(!TO_REGS || Ns * R == M, "only the last pass can leave its outputs in registers");
     float2 x[G][R];
     if constexpr (FROM_REGS) {
 #pragma unroll
@@ -234,18 +238,28 @@ __device__ __forceinline__ void stockham_pass(float2* __restrict__ line, const f
             }
         }
         dft<R, DIR>(x[g]);
-        const int e0 = (j / Ns) * Ns * R + (j & (Ns - 1));
-        float2* wb = line + WriteSlots<R, Ns>::base(j);
+        if constexpr (TO_REGS) {
+            // output r of butterfly j is element j + r * NB = lane + 64 (g + r G)
 #pragma unroll
-        for (int r = 0; r < R; ++r) wb[WriteSlots<R, Ns>::offset(r, e0)] = x[g][r];
+            for (int r = 0; r < R; ++r) out[g + r * G] = x[g][r];
+        } else {
+            const int e0 = (j / Ns) * Ns * R + (j & (Ns - 1));
+            float2* wb = line + WriteSlots<R, Ns>::base(j);
+#pragma unroll
+            for (int r = 0; r < R; ++r) wb[WriteSlots<R, Ns>::offset(r, e0)] = x[g][r];
+        }
     }
 }
 
 template <int M>
 struct FftPlan;
 
+// run<DIR, FROM_REGS, TO_REGS>(line, v, out, lane): the transform of the line
+// (from v[] if FROM_REGS, else from LDS), result in LDS in natural order, or in
+// out[] (element lane + 64*i in out[i]) if TO_REGS.  Unnormalised.
 template <>
 struct FftPlan<256> {
+    static constexpr int V = 256 / kWave;
     PassTw<256, 4, 1> t0;
     PassTw<256, 4, 4> t1;
     PassTw<256, 4, 16> t2;
@@ -254,17 +268,18 @@ struct FftPlan<256> {
     {
         t0.load(tw, lane); t1.load(tw, lane); t2.load(tw, lane); t3.load(tw, lane);
     }
-    template <int DIR, bool REGS>
-    __device__ void run(float2* line, const float2 (&v)[4], int lane) const
+    template <int DIR, bool REGS, bool OREGS>
+    __device__ void run(float2* line, const float2 (&v)[V], float2 (&o)[V], int lane) const
     {
-        stockham_pass<256, 4, 1, DIR, REGS>(line, v, t0, lane);
-        stockham_pass<256, 4, 4, DIR, false>(line, v, t1, lane);
-        stockham_pass<256, 4, 16, DIR, false>(line, v, t2, lane);
-        stockham_pass<256, 4, 64, DIR, false>(line, v, t3, lane);
+        stockham_pass<256, 4, 1, DIR, REGS, false>(line, v, o, t0, lane);
+        stockham_pass<256, 4, 4, DIR, false, false>(line, v, o, t1, lane);
+        stockham_pass<256, 4, 16, DIR, false, false>(line, v, o, t2, lane);
+        stockham_pass<256, 4, 64, DIR, false, OREGS>(line, v, o, t3, lane);
     }
 };
 template <>
 struct FftPlan<512> {
+    static constexpr int V = 512 / kWave;
     PassTw<512, 8, 1> t0;
     PassTw<512, 8, 8> t1;
     PassTw<512, 8, 64> t2;
@@ -272,16 +287,17 @@ struct FftPlan<512> {
     {
         t0.load(tw, lane); t1.load(tw, lane); t2.load(tw, lane);
     }
-    template <int DIR, bool REGS>
-    __device__ void run(float2* line, const float2 (&v)[8], int lane) const
+    template <int DIR, bool REGS, bool OREGS>
+    __device__ void run(float2* line, const float2 (&v)[V], float2 (&o)[V], int lane) const
     {
-        stockham_pass<512, 8, 1, DIR, REGS>(line, v, t0, lane);
-        stockham_pass<512, 8, 8, DIR, false>(line, v, t1, lane);
-        stockham_pass<512, 8, 64, DIR, false>(line, v, t2, lane);
+        stockham_pass<512, 8, 1, DIR, REGS, false>(line, v, o, t0, lane);
+        stockham_pass<512, 8, 8, DIR, false, false>(line, v, o, t1, lane);
+        stockham_pass<512, 8, 64, DIR, false, OREGS>(line, v, o, t2, lane);
     }
 };
 template <>
 struct FftPlan<1024> {
+    static constexpr int V = 1024 / kWave;
     PassTw<1024, 16, 1> t0;
     PassTw<1024, 8, 16> t1;
     PassTw<1024, 8, 128> t2;
@@ -289,16 +305,17 @@ struct FftPlan<1024> {
     {
         t0.load(tw, lane); t1.load(tw, lane); t2.load(tw, lane);
     }
-    template <int DIR, bool REGS>
-    __device__ void run(float2* line, const float2 (&v)[16], int lane) const
+    template <int DIR, bool REGS, bool OREGS>
+    __device__ void run(float2* line, const float2 (&v)[V], float2 (&o)[V], int lane) const
     {
-        stockham_pass<1024, 16, 1, DIR, REGS>(line, v, t0, lane);
-        stockham_pass<1024, 8, 16, DIR, false>(line, v, t1, lane);
-        stockham_pass<1024, 8, 128, DIR, false>(line, v, t2, lane);
+        stockham_pass<1024, 16, 1, DIR, REGS, false>(line, v, o, t0, lane);
+        stockham_pass<1024, 8, 16, DIR, false, false>(line, v, o, t1, lane);
+        stockham_pass<1024, 8, 128, DIR, false, OREGS>(line, v, o, t2, lane);
     }
 };
 template <>
 struct FftPlan<2048> {
+    static constexpr int V = 2048 / kWave;
     PassTw<2048, 16, 1> t0;
     PassTw<2048, 16, 16> t1;
     PassTw<2048, 8, 256, false> t2;
@@ -306,31 +323,52 @@ struct FftPlan<2048> {
     {
         t0.load(tw, lane); t1.load(tw, lane); t2.load(tw, lane);
     }
-    template <int DIR, bool REGS>
-    __device__ void run(float2* line, const float2 (&v)[32], int lane) const
+    template <int DIR, bool REGS, bool OREGS>
+    __device__ void run(float2* line, const float2 (&v)[V], float2 (&o)[V], int lane) const
     {
-        stockham_pass<2048, 16, 1, DIR, REGS>(line, v, t0, lane);
-        stockham_pass<2048, 16, 16, DIR, false>(line, v, t1, lane);
-        stockham_pass<2048, 8, 256, DIR, false>(line, v, t2, lane);
+        stockham_pass<2048, 16, 1, DIR, REGS, false>(line, v, o, t0, lane);
+        stockham_pass<2048, 16, 16, DIR, false, false>(line, v, o, t1, lane);
+        stockham_pass<2048, 8, 256, DIR, false, OREGS>(line, v, o, t2, lane);
     }
 };
 
-// Transform a line held in registers (element lane + 64*i in v[i]); result in
-// `line` (LDS, padded slots) in natural order.  PL is FftPlan<M> or any plan
-// with the same interface (run<DIR, FROM_REGS>(line, v, lane)).
+// Helpers over any plan PL with the interface above (FftPlan<M>, FftPlanMfma<M>).
+// Register arrays hold element lane + 64*i in index i; `line` is the LDS line
+// of the calling wavefront (input for the _lds forms, scratch otherwise).
+
+// registers -> LDS (natural order)
 template <int M, int DIR, class PL>
 __device__ __forceinline__ void fft_from_regs(const PL& P, float2* line,
                                               const float2 (&v)[M / kWave], int lane)
 {
-    P.template run<DIR, true>(line, v, lane);
+    float2 unused[M / kWave];
+    P.template run<DIR, true, false>(line, v, unused, lane);
 }
 
-// Transform a line already in LDS, in place.
+// LDS -> LDS, in place
 template <int M, int DIR, class PL>
 __device__ __forceinline__ void fft_in_lds(const PL& P, float2* line, int lane)
 {
-    float2 dummy[M / kWave];
-    P.template run<DIR, false>(line, dummy, lane);
+    float2 unused[M / kWave];
+    P.template run<DIR, false, false>(line, unused, unused, lane);
+}
+
+// LDS -> registers
+template <int M, int DIR, class PL>
+__device__ __forceinline__ void fft_lds_to_regs(const PL& P, float2* line, float2 (&out)[M / kWave],
+                                                int lane)
+{
+    float2 unused[M / kWave];
+    P.template run<DIR, false, true>(line, unused, out, lane);
+}
+
+// registers -> registers
+template <int M, int DIR, class PL>
+__device__ __forceinline__ void fft_regs_to_regs(const PL& P, float2* line,
+                                                 const float2 (&v)[M / kWave],
+                                                 float2 (&out)[M / kWave], int lane)
+{
+    P.template run<DIR, true, true>(line, v, out, lane);
 }
 
 }  // namespace srk

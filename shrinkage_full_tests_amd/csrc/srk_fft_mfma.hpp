@@ -148,8 +148,20 @@ struct FftPlanMfma {
             }
     }
 
+    // Same contract as FftPlan<M>::run; with OREGS the result is read back from
+    // LDS into out[] (the MFMA output layout is not the lane + 64 i layout).
+    template <int DIR, bool REGS, bool OREGS>
+    __device__ void run(float2* line, const float2 (&v)[V], float2 (&out)[V], int lane) const
+    {
+        run_lds<DIR, REGS>(line, v, lane);
+        if constexpr (OREGS) {
+#pragma unroll
+            for (int k = 0; k < V; ++k) out[k] = line[lslot(lane + kWave * k)];
+        }
+    }
+
     template <int DIR, bool REGS>
-    __device__ void run(float2* line, const float2 (&v)[V], int lane) const
+    __device__ void run_lds(float2* line, const float2 (&v)[V], int lane) const
     {
         float2 x[V];
 #pragma unroll

@@ -1,0 +1,57 @@
+"""Every tuning variant of the engine gives the same operator: row-pass band
+heights 16, 8, 4 and small launch chunks, in both precision forms, on whole
+image tiles (N = M) and on overlap-and-save tiles (N > M), against the FP64
+oracle at the tolerance of each form (README.md:8-10)."""
+import numpy as np
+import pytest
+
+import shrinkage_full_tests_amd as srk
+from shrinkage_full_tests_amd import patterns
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp32": (1e-3, 1e-5), "mixed_fp16": (1e-2, 1e-3)}
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+
+
+def host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy().astype(np.float64)
+
+
+def assert_close(a, n, prec, what):
+    tr, ta = TOL[prec]
+    err = np.abs(a - n)
+    lim = np.maximum(ta, tr * np.maximum(np.abs(a), np.abs(n)))
+    bad = err > lim
+    assert not bad.any(), f"{what}: {bad.sum()} violations, max err {err.max():.3e}"
+
+
+@pytest.mark.parametrize("N,M,kw", [(256, 256, 31), (600, 256, 31)])
+@pytest.mark.parametrize("prec", ["fp32", "mixed_fp16"])
+@pytest.mark.parametrize("rows", [16, 8, 4])
+def test_band_heights_and_chunks(orc, N, M, kw, prec, rows):
+    e = (kw - 1) // 2
+    W = N - 2 * e
+    sigma = e / 3.0
+    gx, gy, hx, hy = srk.kernel_spectra_srsrk(kw, sigma, M, with_spatial=True)
+    Kx, Ky = hx.numpy().astype(np.float64), hy.numpy().astype(np.float64)
+    B = 2
+    f = patterns.grf(N, 11, B=B)
+    g = patterns.grf(N, 12, B=B)
+    v = np.random.default_rng(rows).standard_normal((B, W, W)).astype(np.float32)
+    ws = srk.Workspace(N, W, e, M, B_max=B, precision=prec)
+    ws.set_tuning(chunk_tiles=3, p2_rows=rows)
+    h = host(srk.forward_batched(dev(g), dev(f), gx, gy, e, 1.0, ws))
+    lf, lg = srk.adjoint_batched(dev(v), dev(g), dev(f), gx, gy, e, 1.0, ws)
+    lf, lg = host(lf), host(lg)
+    for b in range(B):
+        assert_close(h[b], orc.forward(f[b], g[b], e, Kx, Ky, fft=True), prec, f"h[{b}]")
+        rf, rg = orc.adjoint(v[b], f[b], g[b], e, Kx, Ky, fft=True)
+        assert_close(lf[b], rf, prec, f"lambda_f[{b}]")
+        assert_close(lg[b], rg, prec, f"lambda_g[{b}]")
