@@ -19,11 +19,34 @@ namespace srk {
 namespace {
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
-__device__ __forceinline__ int wave_id() { return threadIdx.x / kWave; }
+// wavefront index in the block, as a scalar (per-wave pointers stay in SGPRs)
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x / kWave); }
+
+// P1 output staging.  A P1 block transforms CB (8 or 4) consecutive tile
+// columns and stages their half spectra in LDS as st[k * (CB + 1) + c]
+// (k <= M/2, c < CB; the odd pitch keeps the lane-per-k writes on distinct
+// banks), then stores them block-wide into the band layout of X1, where the
+// CB columns of one band are CB RY contiguous elements.
+template <int M, int CB>
+constexpr int stage_elems() { return (M / 2 + 1) * (CB + 1); }
 
 // Unpack the transform Z of two real sequences (p + i q) into the half spectra
 // P(k) = (Z(k) + conj Z(M-k))/2 and Q(k) = (Z(k) - conj Z(M-k))/(2i), k <= M/2,
-// stored in X1 band layout at column xp of planeP and column xq of planeQ.
+// staged at column cp of sP and column cq of sQ.
+template <int M, int CB>
+__device__ __forceinline__ void unpack_stage(const float2* line, float2* sP, int cp, float2* sQ,
+                                             int cq, int lane)
+{
+    for (int k = lane; k <= M / 2; k += kWave) {
+        const float2 z = line[lslot(k)];
+        const float2 zm = line[lslot((M - k) & (M - 1))];
+        sP[k * (CB + 1) + cp] = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
+        sQ[k * (CB + 1) + cq] = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
+    }
+}
+
+// Unpack as above, stored straight into X1 band layout (see x1_index) at
+// column xp of planeP and column xq of planeQ.
 template <int M>
 __device__ __forceinline__ void unpack_store(const float2* line, float2* planeP, int xp,
                                              float2* planeQ, int xq, int rs, int lane)
@@ -36,11 +59,35 @@ __device__ __forceinline__ void unpack_store(const float2* line, float2* planeP,
     }
 }
 
-// zero column spectrum (k <= M/2) at column x of an X1 plane
 template <int M>
 __device__ __forceinline__ void zero_store(float2* plane, int x, int rs, int lane)
 {
     for (int k = lane; k <= M / 2; k += kWave) plane[x1_index<M>(k, x, rs)] = make_float2(0.f, 0.f);
+}
+
+template <int M, int CB>
+__device__ __forceinline__ void zero_stage(float2* st, int c, int lane)
+{
+    for (int k = lane; k <= M / 2; k += kWave) st[k * (CB + 1) + c] = make_float2(0.f, 0.f);
+}
+
+// Block-wide store of the staged columns x0 .. x0+CB-1 into an X1 plane (band
+// layout with bands of 2^rs rows, see x1_index).
+template <int M, int CB, int NT>
+__device__ __forceinline__ void store_stage(float2* __restrict__ plane, const float2* st, int x0,
+                                            int rs)
+{
+    constexpr int LCB = CB == 8 ? 3 : 2;
+    static_assert(CB == 8 || CB == 4, "staged column groups of 8 or 4");
+    const int RY = 1 << rs;
+    // full bands: element e = (band b, column c, row r), e = b CB RY + c RY + r
+    for (int e = threadIdx.x; e < CB * (M / 2); e += NT) {
+        const int b = e >> (rs + LCB), c = (e >> rs) & (CB - 1), r = e & (RY - 1);
+        plane[((b * M + x0) << rs) + (e & (CB * RY - 1))] = st[(b * RY + r) * (CB + 1) + c];
+    }
+    // the last band holds the Nyquist row k = M/2 only
+    if (threadIdx.x < CB)
+        plane[x1_index<M>(M / 2, x0 + threadIdx.x, rs)] = st[(M / 2) * (CB + 1) + threadIdx.x];
 }
 
 // Inverse transform of two Hermitian half spectra P, Q (X2 columns; either may
@@ -89,7 +136,8 @@ __device__ __forceinline__ float cot_at(const float* v, int N, int e, int W, int
 }
 
 // ---------------------------------------------------------------------------
-// P1: real FFT along y of the convolved fields; one wavefront per column pair.
+// P1: real FFT along y of the convolved fields; one wavefront per column pair,
+// 8 columns per block.
 // ---------------------------------------------------------------------------
 template <int M, class PL>
 __global__ __launch_bounds__(256) void k_p1_cols(EngineArgs a, int nf)
@@ -97,52 +145,59 @@ __global__ __launch_bounds__(256) void k_p1_cols(EngineArgs a, int nf)
     constexpr int V = M / kWave;
     constexpr int HP = spec_pitch(M);
     __shared__ float2 lds[4][line_slots(M)];
+    __shared__ float2 st[stage_elems<M, 8>()];
     const int lane = lane_id(), w = wave_id();
     PL FP;
     FP.load(a.tw, lane);
     const int slot = blockIdx.y;
     const Tile t = a.tiles[slot];
     const int N = a.N;
-    const int xa = (blockIdx.x * 4 + w) * 2;
+    const int x0 = blockIdx.x * 8;
+    const int xa = x0 + 2 * w;
     const int ca = t.p0x + xa, cb = ca + 1;
     const bool va = ca >= 0 && ca < N, vb = cb >= 0 && cb < N;
     float2* line = lds[w];
     for (int fi = 0; fi < nf; ++fi) {
-        float2* plane = a.X1 + ((long long)slot * a.np1 + fi) * M * HP;
         if (!va && !vb) {
-            zero_store<M>(plane, xa, a.ry_log2, lane);
-            zero_store<M>(plane, xa + 1, a.ry_log2, lane);
-            continue;
-        }
-        const float* src = (fi == 0 ? a.g0 : a.g1) + (long long)t.b * a.img_stride;
-        // clamped addresses keep every load in bounds; masks zero the padding
-        const float* colA = src + (long long)min(max(ca, 0), N - 1) * N;
-        const float* colB = src + (long long)min(max(cb, 0), N - 1) * N;
-        float2 v[V];
-        if (va && vb && t.p0y >= 0 && t.p0y + M <= N) {
-#pragma unroll
-            for (int i = 0; i < V; ++i) {
-                const int row = t.p0y + lane + kWave * i;
-                v[i] = make_float2(colA[row], colB[row]);
-            }
+            zero_stage<M, 8>(st, 2 * w, lane);
+            zero_stage<M, 8>(st, 2 * w + 1, lane);
         } else {
+            const float* src = (fi == 0 ? a.g0 : a.g1) + (long long)t.b * a.img_stride;
+            // clamped addresses keep every load in bounds; masks zero the padding
+            const float* colA = src + (long long)min(max(ca, 0), N - 1) * N;
+            const float* colB = src + (long long)min(max(cb, 0), N - 1) * N;
+            float2 v[V];
+            if (va && vb && t.p0y >= 0 && t.p0y + M <= N) {
 #pragma unroll
-            for (int i = 0; i < V; ++i) {
-                const int row = t.p0y + lane + kWave * i;
-                const bool rv = row >= 0 && row < N;
-                const int rc = min(max(row, 0), N - 1);
-                const float xa = colA[rc], xb = colB[rc];
-                v[i] = make_float2((va && rv) ? xa : 0.f, (vb && rv) ? xb : 0.f);
+                for (int i = 0; i < V; ++i) {
+                    const int row = t.p0y + lane + kWave * i;
+                    v[i] = make_float2(colA[row], colB[row]);
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < V; ++i) {
+                    const int row = t.p0y + lane + kWave * i;
+                    const bool rv = row >= 0 && row < N;
+                    const int rc = min(max(row, 0), N - 1);
+                    const float pa = colA[rc], pb = colB[rc];
+                    v[i] = make_float2((va && rv) ? pa : 0.f, (vb && rv) ? pb : 0.f);
+                }
             }
+            fft_from_regs<M, -1>(FP, line, v, lane);
+            unpack_stage<M, 8>(line, st, 2 * w, st, 2 * w + 1, lane);
         }
-        fft_from_regs<M, -1>(FP, line, v, lane);
-        unpack_store<M>(line, plane, xa, plane, xa + 1, a.ry_log2, lane);
+        __syncthreads();
+        store_stage<M, 8, 256>(a.X1 + ((long long)slot * a.np1 + fi) * M * HP, st, x0, a.ry_log2);
+        __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------
 // P1A: adjoint source a_d = v0 * d_d f0 (+ v1 * d_d f1) on the valid region,
-// transformed along y as a_x + i a_y; one wavefront per column.
+// transformed along y as a_x + i a_y; one wavefront per column.  Inside the
+// valid region every stencil neighbour lies inside the image (e >= 1), so the
+// loads need no bounds checks: rows outside it load at a clamped row and are
+// masked.
 // ---------------------------------------------------------------------------
 template <int M, class PL>
 __global__ __launch_bounds__(256) void k_p1_adj_src(EngineArgs a, int nf, int plane0)
@@ -163,26 +218,28 @@ __global__ __launch_bounds__(256) void k_p1_adj_src(EngineArgs a, int nf, int pl
         zero_store<M>(planeY, x, a.ry_log2, lane);
         return;
     }
-    const float* f0 = a.f0 + (long long)t.b * a.img_stride;
-    const float* v0 = a.v0 + (long long)t.b * a.w_stride;
-    const float* f1 = nf > 1 ? a.f1 + (long long)t.b * a.img_stride : nullptr;
-    const float* v1 = nf > 1 ? a.v1 + (long long)t.b * a.w_stride : nullptr;
     float2 v[V];
 #pragma unroll
-    for (int i = 0; i < V; ++i) {
-        const int gy = t.p0y + lane + kWave * i;
-        float ax = 0.f, ay = 0.f;
-        if (gy >= e && gy < N - e) {
-            const float c0 = v0[(long long)(gx - e) * W + (gy - e)];
-            ax = c0 * grad_x(f0, N, gy, gx);
-            ay = c0 * grad_y(f0, N, gy, gx);
-            if (nf > 1) {
-                const float c1 = v1[(long long)(gx - e) * W + (gy - e)];
-                ax += c1 * grad_x(f1, N, gy, gx);
-                ay += c1 * grad_y(f1, N, gy, gx);
-            }
+    for (int i = 0; i < V; ++i) v[i] = make_float2(0.f, 0.f);
+    for (int fi = 0; fi < nf; ++fi) {
+        // column pointers are wavefront-uniform; row offsets are unsigned 32-bit
+        const float* fl = (fi == 0 ? a.f0 : a.f1) + (long long)t.b * a.img_stride +
+                          (long long)(gx - 1) * N;
+        const float* fc = fl + N;
+        const float* fr = fc + N;
+        const float* vc = (fi == 0 ? a.v0 : a.v1) + (long long)t.b * a.w_stride +
+                          (long long)(gx - e) * W - e;
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            const int gy = t.p0y + lane + kWave * i;
+            const bool ok = gy >= e && gy < N - e;
+            const unsigned yy = ok ? gy : e;
+            const float c = vc[yy];
+            const float dxf = 0.5f * (fr[yy] - fl[yy]);
+            const float dyf = 0.5f * (fc[yy + 1u] - fc[yy - 1u]);
+            v[i].x += ok ? c * dxf : 0.f;
+            v[i].y += ok ? c * dyf : 0.f;
         }
-        v[i] = make_float2(ax, ay);
     }
     float2* line = lds[w];
     PL FP;
@@ -438,78 +495,88 @@ __global__ __launch_bounds__(256) void k_p3_fwd(EngineArgs a, int nf)
 }
 
 // ---------------------------------------------------------------------------
-// P3 adjoint: per wavefront a pair of output columns (gx, gx+1).
+// P3 adjoint.  A block owns 2 * P3A_WAVES consecutive output columns, a
+// wavefront the pair (gx, gx+1):
 //   lambda_f(c, r) = (w_x(c-1, r) - w_x(c+1, r))/2 + (w_y(c, r-1) - w_y(c, r+1))/2,
 //   w_d = u_d(g) v0 (+ u_d(dg) v1), zero outside the valid region;
 //   lambda_g = inverse y-FFT of the correlation plane.
-// X2 column index of global column c is c - ox0 + 1.
+// Each wavefront transforms u_x + i u_y once per own column; w_x goes to a
+// block array in LDS so that neighbours read it instead of transforming u_x
+// again, and the last wavefront also transforms the two halo columns of the
+// block (u_x of the columns left and right of it, packed in one transform).
+// X2 column index of global column c is c - ox0 + 1; cnt + 2 columns are
+// stored, and columns beyond are zero.
 // ---------------------------------------------------------------------------
+constexpr int P3A_WAVES = 4;
+
 template <int M, class PL>
-__global__ __launch_bounds__(256) void k_p3_adj(EngineArgs a, int nf, int with_u, int planeL)
+__global__ __launch_bounds__(P3A_WAVES * kWave) void k_p3_adj(EngineArgs a, int nf, int with_u,
+                                                              int planeL)
 {
     constexpr int V = M / kWave;
     constexpr int HP = spec_pitch(M);
-    __shared__ float2 lds[4][line_slots(M)];
+    constexpr int CBK = 2 * P3A_WAVES;  // output columns per block
+    __shared__ float2 lds[P3A_WAVES][line_slots(M)];
+    __shared__ float wx[(CBK + 2) * M];  // w_x of block columns -1 .. CBK, rows y
     const int lane = lane_id(), w = wave_id();
     const int slot = blockIdx.y;
     const Tile t = a.tiles[slot];
-    const int j0 = (blockIdx.x * 4 + w) * 2;
+    const int jb = blockIdx.x * CBK;
     const int cnt = t.ox1 - t.ox0;
-    if (j0 >= cnt) return;
+    if (jb >= cnt) return;  // whole block beyond the tile's columns
+    const int j0 = jb + 2 * w;
     PL FP;
     FP.load(a.tw, lane);
-    const bool second = j0 + 1 < cnt;
     const int N = a.N, e = a.e, W = a.W;
     const int gx = t.ox0 + j0;
     float2* line = lds[w];
     const long long pstride = (long long)a.S2 * HP;
     const float2* tbase = a.X2 + (long long)slot * a.np2 * pstride;
-    // X2 column of global column c is c - ox0 + 1; cnt + 2 columns are stored.
-    const int xi0 = j0 + 1;
     const int ncols = cnt + 2;
-    auto col = [&](int plane, int dc) -> const float2* {
-        const int xi2 = xi0 + dc;
+    // X2 column spectrum of output column j (tile-local), or null beyond the stored ones
+    auto col = [&](int plane, int j) -> const float2* {
+        const int xi2 = j + 1;
         return xi2 < ncols ? tbase + plane * pstride + (long long)xi2 * HP : nullptr;
     };
     const long long ib = (long long)t.b * a.img_stride;
 
     if (with_u) {
-        // lambda_f at columns gx (acc0) and gx+1 (acc1) for tile-local rows
-        // y = lane + 64 j, accumulated transform by transform.
+        // lambda_f at columns gx (acc0) and gx+1 (acc1), tile rows y = lane + 64 j
         float acc0[V], acc1[V];
 #pragma unroll
         for (int j = 0; j < V; ++j) acc0[j] = acc1[j] = 0.f;
         for (int fi = 0; fi < nf; ++fi) {
             const float* vv = (fi == 0 ? a.v0 : a.v1) + (long long)t.b * a.w_stride;
-            // cotangent columns gx-1 .. gx+2 at this lane's rows (zero off the valid region)
-            float cv[4][V];
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const int cc = gx - 1 + c;
-                const bool colv = cc >= e && cc < N - e;
-                const float* vcol = vv + (long long)(cc - e) * W - e;
+            // cotangent of global column c at this lane's rows, zero off the valid region
+            auto cot = [&](int c, float (&cv)[V]) {
+                const bool colv = c >= e && c < N - e;
+                const float* vcol = vv + (long long)(colv ? c - e : 0) * W - e;
 #pragma unroll
                 for (int j = 0; j < V; ++j) {
                     const int gy = t.p0y + lane + kWave * j;
-                    cv[c][j] = (colv && gy >= e && gy < N - e) ? vcol[gy] : 0.f;
+                    cv[j] = (colv && gy >= e && gy < N - e) ? vcol[gy] : 0.f;
                 }
-            }
+            };
             const int px = 2 * fi, py = 2 * fi + 1;
             float2 u[V];
-            // u_x at gx-1, gx+1 -> x-stencil transpose at gx
-            ifft_pair<M, PL>(FP, line, col(px, -1), col(px, 1), u, lane);
+            float wy0[V], cv[V];
+            // own columns: u_x + i u_y, w_x to LDS, w_y kept
+            cot(gx, cv);
+            ifft_pair<M, PL>(FP, line, col(px, j0), col(py, j0), u, lane);
 #pragma unroll
-            for (int j = 0; j < V; ++j) acc0[j] += 0.5f * (u[j].x * cv[0][j] - u[j].y * cv[2][j]);
-            // u_x at gx, gx+2 -> x-stencil transpose at gx+1
-            ifft_pair<M, PL>(FP, line, col(px, 0), col(px, 2), u, lane);
+            for (int j = 0; j < V; ++j) {
+                wx[(2 * w + 1) * M + lane + kWave * j] = u[j].x * cv[j];
+                wy0[j] = u[j].y * cv[j];
+            }
+            cot(gx + 1, cv);
+            ifft_pair<M, PL>(FP, line, col(px, j0 + 1), col(py, j0 + 1), u, lane);
 #pragma unroll
-            for (int j = 0; j < V; ++j) acc1[j] += 0.5f * (u[j].x * cv[1][j] - u[j].y * cv[3][j]);
-            // u_y at gx, gx+1: w_y = u_y v written to the line, then the y-stencil
-            // transpose reads the neighbouring rows (same wavefront, in order).
-            ifft_pair<M, PL>(FP, line, col(py, 0), col(py, 1), u, lane);
-#pragma unroll
-            for (int j = 0; j < V; ++j)
-                line[lslot(lane + kWave * j)] = make_float2(u[j].x * cv[1][j], u[j].y * cv[2][j]);
+            for (int j = 0; j < V; ++j) {
+                wx[(2 * w + 2) * M + lane + kWave * j] = u[j].x * cv[j];
+                // w_y of both columns in the line; the y-stencil transpose reads
+                // the neighbouring rows (same wavefront, in order)
+                line[lslot(lane + kWave * j)] = make_float2(wy0[j], u[j].y * cv[j]);
+            }
 #pragma unroll
             for (int j = 0; j < V; ++j) {
                 const int y = lane + kWave * j;
@@ -518,27 +585,53 @@ __global__ __launch_bounds__(256) void k_p3_adj(EngineArgs a, int nf, int with_u
                 acc0[j] += 0.5f * (wm.x - wp.x);
                 acc1[j] += 0.5f * (wm.y - wp.y);
             }
-        }
-        float* o0 = a.out0 + ib + (long long)gx * N;
+            if (w == P3A_WAVES - 1) {
+                // halo columns jb - 1 and jb + CBK: u_x only
+                float cr[V];
+                cot(t.ox0 + jb - 1, cv);
+                cot(t.ox0 + jb + CBK, cr);
+                ifft_pair<M, PL>(FP, line, col(px, jb - 1), col(px, jb + CBK), u, lane);
 #pragma unroll
-        for (int j = 0; j < V; ++j) {
-            const int gy = t.p0y + lane + kWave * j;
-            if (gy >= t.oy0 && gy < t.oy1) {
-                o0[gy] = acc0[j];
-                if (second) o0[N + gy] = acc1[j];
+                for (int j = 0; j < V; ++j) {
+                    wx[lane + kWave * j] = u[j].x * cv[j];
+                    wx[(CBK + 1) * M + lane + kWave * j] = u[j].y * cr[j];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                const int y = lane + kWave * j;
+                acc0[j] += 0.5f * (wx[(2 * w) * M + y] - wx[(2 * w + 2) * M + y]);
+                acc1[j] += 0.5f * (wx[(2 * w + 1) * M + y] - wx[(2 * w + 3) * M + y]);
+            }
+            __syncthreads();
+        }
+        if (j0 < cnt) {
+            const bool second = j0 + 1 < cnt;
+            float* o0 = a.out0 + ib + (long long)gx * N;
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                const int gy = t.p0y + lane + kWave * j;
+                if (gy >= t.oy0 && gy < t.oy1) {
+                    o0[gy] = acc0[j];
+                    if (second) o0[N + gy] = acc1[j];
+                }
             }
         }
     }
     // lambda_g
-    float2 l[V];
-    ifft_pair<M, PL>(FP, line, col(planeL, 0), col(planeL, 1), l, lane);
-    float* o1 = a.out1 + ib + (long long)gx * N;
+    if (j0 < cnt) {
+        const bool second = j0 + 1 < cnt;
+        float2 l[V];
+        ifft_pair<M, PL>(FP, line, col(planeL, j0), col(planeL, j0 + 1), l, lane);
+        float* o1 = a.out1 + ib + (long long)gx * N;
 #pragma unroll
-    for (int j = 0; j < V; ++j) {
-        const int gy = t.p0y + lane + kWave * j;
-        if (gy >= t.oy0 && gy < t.oy1) {
-            o1[gy] = l[j].x;
-            if (second) o1[N + gy] = l[j].y;
+        for (int j = 0; j < V; ++j) {
+            const int gy = t.p0y + lane + kWave * j;
+            if (gy >= t.oy0 && gy < t.oy1) {
+                o1[gy] = l[j].x;
+                if (second) o1[N + gy] = l[j].y;
+            }
         }
     }
 }
@@ -656,8 +749,8 @@ int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s, 
         a.tiles = a0.tiles + t0;
         a.ry_log2 = RY == 16 ? 4 : RY == 8 ? 3 : 2;
         const dim3 g1(M / 8, c), g1a(M / 4, c), g2(P2Geom<M, RY>::blocks(c));
-        const dim3 g3f((cols_out + 3) / 4, c), g3a(((cols_out + 1) / 2 + 3) / 4, c);
-        const dim3 b256(256), b2(NW * kWave);
+        const dim3 g3f((cols_out + 3) / 4, c), g3a((cols_out + 2 * P3A_WAVES - 1) / (2 * P3A_WAVES), c);
+        const dim3 b256(256), b2(NW * kWave), b3a(P3A_WAVES * kWave);
         // launch one pass, bracketed by timing events when a timer is given
         auto run = [&](int kind, auto&& launch) {
             hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -680,7 +773,7 @@ int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s, 
         };
         auto p3f = [&](int nf) { run(kP3Fwd, [&] { k_p3_fwd<M, PL><<<g3f, b256, 0, s>>>(a, nf); }); };
         auto p3a = [&](int nf, int with_u, int planeL) {
-            run(kP3Adj, [&] { k_p3_adj<M, PL><<<g3a, b256, 0, s>>>(a, nf, with_u, planeL); });
+            run(kP3Adj, [&] { k_p3_adj<M, PL><<<g3a, b3a, 0, s>>>(a, nf, with_u, planeL); });
         };
         switch (op) {
         case Op::Forward:
