@@ -66,7 +66,7 @@ def cpu_baseline(N, kw, sigma, e, budget_s=12.0):
     t0 = time.perf_counter()
     oracle.fwd_adj_fft_batch(f, f, v, e, Kx, Ky)
     t1 = time.perf_counter() - t0
-    n = max(1, min(64, int(budget_s / max(t1, 1e-3))))
+    n = max(1, min(400, int(budget_s / max(t1, 1e-3))))
     fb = np.repeat(f, n, axis=0)
     vb = np.repeat(v, n, axis=0)
     t0 = time.perf_counter()

@@ -45,7 +45,8 @@ enum {
     SRK_ERR_BAD_DIMS = -2,         /* W != N - 2*erosion, W <= 0, erosion < 1   */
     SRK_ERR_NULL_PTR = -3,         /* a required pointer is NULL                */
     SRK_ERR_MISALIGNED = -4,       /* pointer not aligned to its element type   */
-    SRK_ERR_UNSUPPORTED_SIZE = -5, /* tile/FFT size M not in {256,512,1024,2048}*/
+    SRK_ERR_UNSUPPORTED_SIZE = -5, /* tile/FFT size M not in {256,512,1024,2048},
+                                      or a CD signal beyond the kernel's limits */
     SRK_ERR_WORKSPACE = -6,        /* workspace made for other N / W / erosion  */
     SRK_ERR_BATCH = -7,            /* B < 1 or B > workspace B_max              */
     SRK_ERR_HIP = -8,              /* HIP runtime error (see srkLastHipError)   */
@@ -129,7 +130,7 @@ int srkWorkspaceInfo(srk_workspace_t ws, int* M, int* fwd_tiles_per_image,
 int srkWorkspaceSetTiming(srk_workspace_t ws, int enable);
 int srkWorkspaceKernelTimes(srk_workspace_t ws, double* total_ms, int* launches,
                             long long* tiles);
-const char* srkKernelKindName(int kind);  /* "k_p1_cols", ..., or "" */
+const char* srkKernelKindName(int kind);  /* "k_p1_cols", ..., "k_p3_adj", or "" */
 
 int srkSynchronize(hipStream_t stream);
 
@@ -212,6 +213,37 @@ int srkTangentAdjointBatched(const float* f, const float* g, const float* df, co
                              const void* Gy_freq, int N, int W, int erosion, float dx, int B,
                              hipStream_t stream, float* dLam_f, float* dLam_g,
                              srk_workspace_t ws);
+
+/* ---- critical-dimension (CD) measurement -------------------------------- */
+
+/*
+ * Replaces MeasureCD_Batched (third_party/src/metricUtil/cd_measure_util.cu:310-446,
+ * declared at third_party/include/metricUtil/cd_measure_util.cuh:74-90): for B
+ * row-major signals in [B x N] (device), threshold subtracted, upsample by L
+ * with a 2A+1-tap sinc x Blackman polyphase filter, find the nearest crossings
+ * left of and at-or-right of the centre (N L - 1)/2 and write, per signal,
+ * left and right edge in pixels and cd = right - left (device arrays [B]; a
+ * missing edge is -inf / +inf).  The reference returns a host vector and
+ * synchronises; this call is asynchronous on the stream and allocates
+ * nothing.  threshold = 0 measures zero crossings as the reference does.
+ * Limits: N * L <= 2^24 and N + L(2A+1) + 512 floats within 160 KB of LDS.
+ */
+int srkMeasureCDBatched(const float* d_in, int B, int N, int L, int A, float threshold,
+                        float* d_left_px, float* d_right_px, float* d_cd_px,
+                        hipStream_t stream);
+
+/*
+ * CD of the shrinkage result, fused: signal = E_crop + coeff_c * R on the centre
+ * row (center_row != 0: row W/2 across the columns) or the centre column of
+ * the W x W valid region, then as srkMeasureCDBatched with N = W.  E is
+ * [B x N x N] and R [B x W x W], column-major; replaces crop_center +
+ * apply_shrinkage + extract_center_row/col + MeasureCD_Batched of
+ * training/main_cd_training.cu:128-165,213-261.
+ */
+int srkShrinkageCDBatched(const float* d_E, const float* d_R, int N, int W, int erosion, int B,
+                          float coeff_c, int center_row, int L, int A, float threshold,
+                          float* d_left_px, float* d_right_px, float* d_cd_px,
+                          hipStream_t stream);
 
 #ifdef __cplusplus
 }

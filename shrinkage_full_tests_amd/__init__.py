@@ -19,6 +19,7 @@ __all__ = [
     "srkTrainingForward", "srkForwardOaS", "adjointSRK", "shrinkageTangentOaS",
     "shrinkageTangentAdjointOaS", "forward_batched", "adjoint_batched", "tangent_batched",
     "tangent_adjoint_batched", "ShrinkageFunction", "erosion_for", "SUPPORTED_M",
+    "measure_cd_batched", "shrinkage_cd_batched",
 ]
 
 SUPPORTED_M = (256, 512, 1024, 2048)
@@ -233,6 +234,33 @@ def tangent_adjoint_batched(f, g, df, dg, lam_h, dlam_h, Gx_freq, Gy_freq, erosi
          _ptr(dlam_h), _ptr(Gx_freq), _ptr(Gy_freq), N, W, erosion, float(dx), B,
          _stream(stream), _ptr(dLam_f), _ptr(dLam_g), ws.handle)
     return dLam_f, dLam_g
+
+
+# ---- critical-dimension measurement ------------------------------------------
+
+def measure_cd_batched(sig, L: int = 16, A: int = 8, threshold: float = 0.0, stream=None):
+    """MeasureCD_Batched of [B, N] row-major device signals: (left_px, right_px,
+    cd_px) device tensors [B]; a missing edge is -inf / +inf."""
+    torch = _torch()
+    sig = sig.contiguous()
+    B, N = sig.shape
+    out = [torch.empty(B, dtype=torch.float32, device=sig.device) for _ in range(3)]
+    call("srkMeasureCDBatched", _ptr(sig), B, N, L, A, float(threshold), *map(_ptr, out),
+         _stream(stream))
+    return tuple(out)
+
+
+def shrinkage_cd_batched(E, R, erosion: int, coeff_c: float = 1.0, center_row: bool = True,
+                         L: int = 16, A: int = 8, threshold: float = 0.0, stream=None):
+    """CD of E_crop + coeff_c R on the centre row (or column) of the W x W valid
+    region; E [B, N, N], R [B, W, W] device tensors in the reference layout."""
+    torch = _torch()
+    B, N = E.shape[0], E.shape[-1]
+    W = R.shape[-1]
+    out = [torch.empty(B, dtype=torch.float32, device=E.device) for _ in range(3)]
+    call("srkShrinkageCDBatched", _ptr(E), _ptr(R), N, W, erosion, B, float(coeff_c),
+         int(bool(center_row)), L, A, float(threshold), *map(_ptr, out), _stream(stream))
+    return tuple(out)
 
 
 def _make_function():

@@ -44,6 +44,8 @@ SIGNATURES = {
     "srkTangentBatched": (_i, [_p, _p, _p, _p, _p, _p, _i, _i, _i, _f, _i, _p, _p, _p]),
     "srkTangentAdjointBatched": (_i, [_p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _f, _i, _p,
                                       _p, _p, _p]),
+    "srkMeasureCDBatched": (_i, [_p, _i, _i, _i, _i, _f, _p, _p, _p, _p]),
+    "srkShrinkageCDBatched": (_i, [_p, _p, _i, _i, _i, _i, _f, _i, _i, _i, _f, _p, _p, _p, _p]),
 }
 
 

@@ -3,6 +3,7 @@
 #include "srk.h"
 
 #include "srk_engine.hpp"
+#include "srk_cd.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -302,6 +303,7 @@ srk::EngineArgs base_args(const srk_workspace_s* w, float dx)
     a.w_stride = (long long)w->W * w->W;
     a.GT = w->GT;
     a.tw = w->tw;
+    a.zcol = w->tw + w->M + 512;
     a.X1 = w->X1;
     a.X2 = w->X2;
     a.p2_rows = w->p2_rows;
@@ -491,8 +493,9 @@ int srkWorkspaceCreate(int N, int W, int erosion, int M, int B_max, size_t scrat
     w->x1_bytes = cmax * x1_per;
     w->x2_bytes = cmax * x2_per;
     // twiddles exp(-2 pi i k / M), followed by the per-lane FP16 hi/lo tiles of
-    // the 16-point DFT used by the matrix-core transforms (256 x 8 halves).
-    std::vector<float2> tw(M + 512);
+    // the 16-point DFT used by the matrix-core transforms (256 x 8 halves), and
+    // one all-zero column spectrum (the stand-in for columns past a tile's end).
+    std::vector<float2> tw(M + 512 + srk::spec_pitch(M), make_float2(0.f, 0.f));
     for (int k = 0; k < M; ++k) {
         const double a = -2.0 * M_PI * k / M;
         tw[k] = make_float2((float)std::cos(a), (float)std::sin(a));
@@ -738,6 +741,30 @@ int shrinkageTangentAdjointOaS(const float* f, const float* g, const float* df,
 {
     return srkTangentAdjointBatched(f, g, df, dg, lam_h, dlam_h, Gx, Gy, N, W, erosion, dx, 1,
                                     stream, dLam_f, dLam_g, w);
+}
+
+int srkMeasureCDBatched(const float* d_in, int B, int N, int L, int A, float threshold,
+                        float* d_left_px, float* d_right_px, float* d_cd_px, hipStream_t stream)
+{
+    if (!d_in || !d_left_px || !d_right_px || !d_cd_px) return SRK_ERR_NULL_PTR;
+    if (B < 1) return SRK_ERR_BATCH;
+    if (N < 2 || L < 1 || A < 1) return SRK_ERR_BAD_DIMS;
+    if (!srk::cd_shape_ok(N, L, A)) return SRK_ERR_UNSUPPORTED_SIZE;
+    return hip_status(srk::cd_measure_rows(d_in, B, N, L, A, threshold, d_left_px, d_right_px,
+                                           d_cd_px, stream));
+}
+
+int srkShrinkageCDBatched(const float* d_E, const float* d_R, int N, int W, int erosion, int B,
+                          float coeff_c, int center_row, int L, int A, float threshold,
+                          float* d_left_px, float* d_right_px, float* d_cd_px,
+                          hipStream_t stream)
+{
+    if (!d_E || !d_R || !d_left_px || !d_right_px || !d_cd_px) return SRK_ERR_NULL_PTR;
+    if (B < 1) return SRK_ERR_BATCH;
+    if (W != N - 2 * erosion || W < 2 || erosion < 0 || L < 1 || A < 1) return SRK_ERR_BAD_DIMS;
+    if (!srk::cd_shape_ok(W, L, A)) return SRK_ERR_UNSUPPORTED_SIZE;
+    return hip_status(srk::cd_measure_shrink(d_E, d_R, N, W, erosion, B, coeff_c, center_row, L, A,
+                                             threshold, d_left_px, d_right_px, d_cd_px, stream));
 }
 
 }  // extern "C"

@@ -18,6 +18,11 @@ namespace srk {
 
 namespace {
 
+template <class PL>
+struct IsMfmaPlan { static constexpr bool value = false; };
+template <int M>
+struct IsMfmaPlan<FftPlanMfma<M>> { static constexpr bool value = true; };
+
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 // wavefront index in the block, as a scalar (per-wave pointers stay in SGPRs)
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x / kWave); }
@@ -29,6 +34,21 @@ __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane
 // CB columns of one band are CB RY contiguous elements.
 template <int M, int CB>
 constexpr int stage_elems() { return (M / 2 + 1) * (CB + 1); }
+
+// base[idx] for a wavefront-uniform base and a per-lane index whose byte
+// offset fits in 32 bits: the address becomes base + zext(offset), which the
+// compiler emits as a scalar-base load with a 32-bit vector offset.
+template <class T>
+__device__ __forceinline__ T ldu(const T* base, unsigned idx)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                       idx * (unsigned)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ void stu(T* base, unsigned idx, T v)
+{
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + idx * (unsigned)sizeof(T)) = v;
+}
 
 // Unpack the transform Z of two real sequences (p + i q) into the half spectra
 // P(k) = (Z(k) + conj Z(M-k))/2 and Q(k) = (Z(k) - conj Z(M-k))/(2i), k <= M/2,
@@ -54,15 +74,15 @@ __device__ __forceinline__ void unpack_store(const float2* line, float2* planeP,
     for (int k = lane; k <= M / 2; k += kWave) {
         const float2 z = line[lslot(k)];
         const float2 zm = line[lslot((M - k) & (M - 1))];
-        planeP[x1_index<M>(k, xp, rs)] = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
-        planeQ[x1_index<M>(k, xq, rs)] = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
+        stu(planeP, x1_index<M>(k, xp, rs), make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y)));
+        stu(planeQ, x1_index<M>(k, xq, rs), make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x)));
     }
 }
 
 template <int M>
 __device__ __forceinline__ void zero_store(float2* plane, int x, int rs, int lane)
 {
-    for (int k = lane; k <= M / 2; k += kWave) plane[x1_index<M>(k, x, rs)] = make_float2(0.f, 0.f);
+    for (int k = lane; k <= M / 2; k += kWave) stu(plane, x1_index<M>(k, x, rs), make_float2(0.f, 0.f));
 }
 
 template <int M, int CB>
@@ -83,57 +103,78 @@ __device__ __forceinline__ void store_stage(float2* __restrict__ plane, const fl
     // full bands: element e = (band b, column c, row r), e = b CB RY + c RY + r
     for (int e = threadIdx.x; e < CB * (M / 2); e += NT) {
         const int b = e >> (rs + LCB), c = (e >> rs) & (CB - 1), r = e & (RY - 1);
-        plane[((b * M + x0) << rs) + (e & (CB * RY - 1))] = st[(b * RY + r) * (CB + 1) + c];
+        stu(plane, ((b * M + x0) << rs) + (e & (CB * RY - 1)), st[(b * RY + r) * (CB + 1) + c]);
     }
     // the last band holds the Nyquist row k = M/2 only
     if (threadIdx.x < CB)
-        plane[x1_index<M>(M / 2, x0 + threadIdx.x, rs)] = st[(M / 2) * (CB + 1) + threadIdx.x];
+        stu(plane, x1_index<M>(M / 2, x0 + threadIdx.x, rs), st[(M / 2) * (CB + 1) + threadIdx.x]);
 }
 
-// Inverse transform of two Hermitian half spectra P, Q (X2 columns; either may
-// be null for zero) packed as Z = P + iQ; afterwards out[i] = p(y) + i q(y) at
-// tile row y = lane + 64 i.  The DC and Nyquist bins are taken as real, as a
-// real inverse transform does.
+// Two Hermitian half spectra P, Q (X2 columns, or the zero column) as loaded
+// for the lane's transform inputs k = lane + 64 i (raw, so that a load can be
+// issued one transform ahead), and their packing Z = P + iQ.  The DC and
+// Nyquist bins are taken as real, as a real inverse transform does.
+template <int M>
+struct PairRaw {
+    static constexpr int V = M / kWave;
+    float2 p[V], q[V];
+
+    __device__ __forceinline__ void load(const float2* P, const float2* Q, int lane)
+    {
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            const int k = lane + kWave * i;
+            const unsigned kk = k <= M / 2 ? k : M - k;
+            p[i] = ldu(P, kk);
+            q[i] = ldu(Q, kk);
+        }
+    }
+    __device__ __forceinline__ void pack(float2 (&v)[V], int lane) const
+    {
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            const int k = lane + kWave * i;
+            float2 pp = p[i], qq = q[i];
+            if (k == 0 || k == M / 2) {
+                pp.y = 0.f;
+                qq.y = 0.f;
+            }
+            v[i] = k <= M / 2 ? make_float2(pp.x - qq.y, pp.y + qq.x)
+                              : make_float2(pp.x + qq.y, qq.x - pp.y);
+        }
+    }
+};
+
+// Inverse transform of P + iQ; afterwards out[i] = p(y) + i q(y) at tile row
+// y = lane + 64 i.
 template <int M, class PL>
 __device__ __forceinline__ void ifft_pair(const PL& FP, float2* line, const float2* P,
                                           const float2* Q, float2 (&out)[M / kWave], int lane)
 {
-    constexpr int V = M / kWave;
-    float2 v[V];
-#pragma unroll
-    for (int i = 0; i < V; ++i) {
-        const int k = lane + kWave * i;
-        const int kk = k <= M / 2 ? k : M - k;
-        float2 p = P ? P[kk] : make_float2(0.f, 0.f);
-        float2 q = Q ? Q[kk] : make_float2(0.f, 0.f);
-        if (kk == 0 || kk == M / 2) {
-            p.y = 0.f;
-            q.y = 0.f;
-        }
-        v[i] = k <= M / 2 ? make_float2(p.x - q.y, p.y + q.x) : make_float2(p.x + q.y, q.x - p.y);
-    }
+    PairRaw<M> r;
+    r.load(P, Q, lane);
+    float2 v[M / kWave];
+    r.pack(v, lane);
     fft_regs_to_regs<M, +1>(FP, line, v, out, lane);
 }
 
-__device__ __forceinline__ float grad_x(const float* f, int N, int r, int c)
+// Three neighbouring columns gx-1, gx, gx+1 of an image (wavefront-uniform
+// pointers) and the central-difference gradient at row yy, for points whose
+// stencil lies inside the image.
+struct Col3 {
+    const float *l, *c, *r;
+};
+__device__ __forceinline__ Col3 col3(const float* img, int N, int gx)
 {
-    const float right = (c + 1 < N) ? f[(long long)(c + 1) * N + r] : 0.f;
-    const float left = (c - 1 >= 0) ? f[(long long)(c - 1) * N + r] : 0.f;
-    return 0.5f * (right - left);
+    const float* c = img + (long long)gx * N;
+    return {c - N, c, c + N};
 }
-__device__ __forceinline__ float grad_y(const float* f, int N, int r, int c)
+__device__ __forceinline__ float2 grad_in(const Col3& f, unsigned yy)
 {
-    const float down = (r + 1 < N) ? f[(long long)c * N + r + 1] : 0.f;
-    const float up = (r - 1 >= 0) ? f[(long long)c * N + r - 1] : 0.f;
-    return 0.5f * (down - up);
+    return make_float2(0.5f * (ldu(f.r, yy) - ldu(f.l, yy)),
+                       0.5f * (ldu(f.c, yy + 1u) - ldu(f.c, yy - 1u)));
 }
 
-// Cotangent v on the valid region, zero outside (global coordinates).
-__device__ __forceinline__ float cot_at(const float* v, int N, int e, int W, int c, int r)
-{
-    if (c < e || c >= N - e || r < e || r >= N - e) return 0.f;
-    return v[(long long)(c - e) * W + (r - e)];
-}
 
 // ---------------------------------------------------------------------------
 // P1: real FFT along y of the convolved fields; one wavefront per column pair,
@@ -171,7 +212,7 @@ __global__ __launch_bounds__(256) void k_p1_cols(EngineArgs a, int nf)
 #pragma unroll
                 for (int i = 0; i < V; ++i) {
                     const int row = t.p0y + lane + kWave * i;
-                    v[i] = make_float2(colA[row], colB[row]);
+                    v[i] = make_float2(ldu(colA, row), ldu(colB, row));
                 }
             } else {
 #pragma unroll
@@ -179,7 +220,7 @@ __global__ __launch_bounds__(256) void k_p1_cols(EngineArgs a, int nf)
                     const int row = t.p0y + lane + kWave * i;
                     const bool rv = row >= 0 && row < N;
                     const int rc = min(max(row, 0), N - 1);
-                    const float pa = colA[rc], pb = colB[rc];
+                    const float pa = ldu(colA, rc), pb = ldu(colB, rc);
                     v[i] = make_float2((va && rv) ? pa : 0.f, (vb && rv) ? pb : 0.f);
                 }
             }
@@ -234,9 +275,9 @@ __global__ __launch_bounds__(256) void k_p1_adj_src(EngineArgs a, int nf, int pl
             const int gy = t.p0y + lane + kWave * i;
             const bool ok = gy >= e && gy < N - e;
             const unsigned yy = ok ? gy : e;
-            const float c = vc[yy];
-            const float dxf = 0.5f * (fr[yy] - fl[yy]);
-            const float dyf = 0.5f * (fc[yy + 1u] - fc[yy - 1u]);
+            const float c = ldu(vc, yy);
+            const float dxf = 0.5f * (ldu(fr, yy) - ldu(fl, yy));
+            const float dyf = 0.5f * (ldu(fc, yy + 1u) - ldu(fc, yy - 1u));
             v[i].x += ok ? c * dxf : 0.f;
             v[i].y += ok ? c * dyf : 0.f;
         }
@@ -249,34 +290,59 @@ __global__ __launch_bounds__(256) void k_p1_adj_src(EngineArgs a, int nf, int pl
 }
 
 // ---------------------------------------------------------------------------
-// P2 geometry.  A block owns RY consecutive k_y rows (a band) of one tile.
-// The M/(2 RY) full bands come first in launch order, arranged so that block
-// L runs band L mod 8 (+ 8 j) on XCD L mod 8 (workgroups are dispatched to the
-// eight XCDs round robin): every XCD only ever reads its own eighth of the
-// spectra, which stays resident in its L2.  The last band (the Nyquist
-// row k_y = M/2 alone) follows, one block per tile.
+// P2 work items.  An item is one band (RY consecutive k_y rows) of one tile
+// (and, for the conv pass, one field).  Full bands come first in launch
+// order: block L runs on XCD L mod 8 (workgroups are dispatched to the eight
+// XCDs round robin) and only takes bands b with b mod 8 = L mod 8, so every
+// XCD reads only its own eighth of the spectra, which stays resident in its
+// L2.  Each such block takes up to ITEMS consecutive items of its XCD's
+// list (the conv pass prefetches the next item's band into registers while it
+// transforms the current one).  The Nyquist rows (k_y = M/2, a band of one
+// row) follow, one item per block.
 // ---------------------------------------------------------------------------
-template <int M, int RY>
-struct P2Geom {
-    static constexpr int NBF = M / (2 * RY);   // full bands per tile
+template <int M, int RY, int ITEMS>
+struct P2Items {
+    static constexpr int NBF = M / (2 * RY);  // full bands per tile
+    static constexpr int BPX = NBF / 8;       // full bands per tile and XCD
     static_assert(NBF % 8 == 0, "full bands must split evenly over the eight XCDs");
-    static constexpr int blocks(int tiles) { return (NBF + 1) * tiles; }
-    int slot, band;
-    __device__ P2Geom()
+    static constexpr int full_blocks(int tiles, int nf)
+    {
+        return 8 * ((BPX * tiles * nf + ITEMS - 1) / ITEMS);
+    }
+    static constexpr int blocks(int tiles, int nf) { return full_blocks(tiles, nf) + tiles * nf; }
+
+    int xcd, first, count, nf;
+    bool nyquist;
+    __device__ P2Items(int tiles, int nf_) : nf(nf_)
     {
         const int L = blockIdx.x;
-        const int tiles = gridDim.x / (NBF + 1);
-        if (L < NBF * tiles) {
-            constexpr int BPX = NBF / 8;   // bands per XCD
-            const int q = L >> 3;
-            band = (L & 7) + 8 * (q % BPX);
-            slot = q / BPX;
+        const int fb = full_blocks(tiles, nf);
+        nyquist = L >= fb;
+        if (!nyquist) {
+            xcd = L & 7;
+            first = (L >> 3) * ITEMS;
+            count = min(ITEMS, BPX * tiles * nf - first);
         } else {
-            band = NBF;
-            slot = L - NBF * tiles;
+            xcd = 0;
+            first = L - fb;
+            count = 1;
         }
     }
-    __device__ int rows() const { return band < NBF ? RY : 1; }
+    // item k of this block -> tile slot, band, field
+    __device__ void get(int k, int& slot, int& band, int& fi) const
+    {
+        const int i = first + k;
+        fi = i % nf;
+        const int j = i / nf;
+        if (nyquist) {
+            slot = j;
+            band = NBF;
+        } else {
+            slot = j / BPX;
+            band = xcd + 8 * (j % BPX);
+        }
+    }
+    __device__ int rows(int band) const { return band < NBF ? RY : 1; }
 };
 
 // LDS pitch of the P2 band lines: line_slots(M) + 32/RY, so that the
@@ -285,24 +351,33 @@ struct P2Geom {
 template <int M, int RY>
 constexpr int p2_pitch() { return line_slots(M) + 32 / RY; }
 
-// Band of one X1 plane (M x RY elements, contiguous) -> LDS lines: line r holds
-// k_y row k0 + r along x.  Rows >= nrows are zero.
+// Band of one X1 plane (M x RY elements, contiguous) -> registers of the
+// block's threads (fetch_band) -> LDS lines (put_band): line r holds k_y row
+// k0 + r along x.  Rows >= nrows are zero.
 template <int M, int RY, int NW>
-__device__ __forceinline__ void load_band(float2* L, const float2* __restrict__ band, int nrows)
-{
-    constexpr int T = NW * kWave, XS = T / RY, IT = M * RY / T, LP = p2_pitch<M, RY>();
-    static_assert(XS % 8 == 0 && (M * RY) % T == 0, "band load shape");
-    const int r = threadIdx.x % RY, x0 = threadIdx.x / RY;
-    float2* d = L + r * LP + lslot(x0);
-    const float2* src = band + threadIdx.x;
-    const bool ok = r < nrows;
+struct BandRegs {
+    static constexpr int T = NW * kWave, IT = M * RY / T;
+    static_assert((T / RY) % 8 == 0 && (M * RY) % T == 0, "band load shape");
+    float2 v[IT];
+
+    __device__ __forceinline__ void fetch(const float2* __restrict__ band, int nrows)
+    {
+        const bool ok = (int)(threadIdx.x % RY) < nrows;
+        const float2* src = band + threadIdx.x;
 #pragma unroll
-    for (int i = 0; i < IT; ++i) {
-        float2 v = make_float2(0.f, 0.f);
-        if (ok) v = src[i * T];
-        d[i * slot_off(XS)] = v;
+        for (int i = 0; i < IT; ++i) {
+            v[i] = make_float2(0.f, 0.f);
+            if (ok) v[i] = ldu(src, i * T);
+        }
     }
-}
+    __device__ __forceinline__ void put(float2* L) const
+    {
+        constexpr int XS = T / RY, LP = p2_pitch<M, RY>();
+        float2* d = L + (threadIdx.x % RY) * LP + lslot(threadIdx.x / RY);
+#pragma unroll
+        for (int i = 0; i < IT; ++i) d[i * slot_off(XS)] = v[i];
+    }
+};
 
 // LDS lines -> X2 plane ([x][k_y], pitch HP) for tile-local columns
 // [lo, lo + cnt); columns outside [0, M) are stored as zero.
@@ -327,8 +402,10 @@ __device__ __forceinline__ void store_band(float2* __restrict__ plane, const flo
 // P2 conv: x-FFT of each convolved-field band row, products with Gx and Gy in
 // registers, inverse x-FFTs, crop; field fi -> X2 planes 2fi (u_x), 2fi+1 (u_y).
 // ---------------------------------------------------------------------------
+constexpr int P2C_ITEMS = 4;  // items per conv block
+
 template <int M, int RY, int NW, class PL>
-__global__ __launch_bounds__(NW * kWave) void k_p2_conv(EngineArgs a, int nf)
+__global__ __launch_bounds__(NW * kWave) void k_p2_conv(EngineArgs a, int nf, int tiles)
 {
     constexpr int V = M / kWave;
     constexpr int H = M / 2 + 1;
@@ -340,17 +417,26 @@ __global__ __launch_bounds__(NW * kWave) void k_p2_conv(EngineArgs a, int nf)
     const int lane = lane_id(), w = wave_id();
     PL FP;
     FP.load(a.tw, lane);
-    const P2Geom<M, RY> geo;
-    const int slot = geo.slot;
-    const Tile t = a.tiles[slot];
-    const int k0 = geo.band * RY;
-    const int nrows = geo.rows();
-    const int lo = t.ox0 - t.p0x - a.halo2;
-    const int cnt = t.ox1 - t.ox0 + 2 * a.halo2;
-    for (int fi = 0; fi < nf; ++fi) {
-        const float2* src = a.X1 + ((long long)slot * a.np1 + fi) * M * HP + (long long)k0 * M;
-        load_band<M, RY, NW>(A, src, nrows);
+    const P2Items<M, RY, P2C_ITEMS> items(tiles, nf);
+    auto band_src = [&](int slot, int band, int fi) {
+        return a.X1 + ((long long)slot * a.np1 + fi) * M * HP + (long long)band * RY * M;
+    };
+    BandRegs<M, RY, NW> pf;
+    int slot, band, fi;
+    items.get(0, slot, band, fi);
+    pf.fetch(band_src(slot, band, fi), items.rows(band));
+    for (int k = 0; k < items.count; ++k) {
+        items.get(k, slot, band, fi);
+        const int k0 = band * RY;
+        const int nrows = items.rows(band);
+        if (k) __syncthreads();  // the previous item's stores have read A and Bs
+        pf.put(A);
         __syncthreads();
+        if (k + 1 < items.count) {
+            int s2, b2, f2;
+            items.get(k + 1, s2, b2, f2);
+            pf.fetch(band_src(s2, b2, f2), items.rows(b2));
+        }
         for (int r = w; r < nrows; r += NW) {
             float2* la = A + r * LP;
             float2* lb = Bs + r * LP;
@@ -359,8 +445,8 @@ __global__ __launch_bounds__(NW * kWave) void k_p2_conv(EngineArgs a, int nf)
             float2 gxv[V], gyv[V];
 #pragma unroll
             for (int i = 0; i < V; ++i) {
-                gxv[i] = gxr[kWave * i];
-                gyv[i] = gyr[kWave * i];
+                gxv[i] = ldu(gxr, kWave * i);
+                gyv[i] = ldu(gyr, kWave * i);
             }
             float2 X[V], U[V];
             fft_lds_to_regs<M, -1>(FP, la, X, lane);
@@ -373,11 +459,12 @@ __global__ __launch_bounds__(NW * kWave) void k_p2_conv(EngineArgs a, int nf)
             fft_from_regs<M, +1>(FP, lb, U, lane);
         }
         __syncthreads();
+        const Tile t = a.tiles[slot];
+        const int lo = t.ox0 - t.p0x - a.halo2;
+        const int cnt = t.ox1 - t.ox0 + 2 * a.halo2;
         float2* dstA = a.X2 + ((long long)slot * a.np2 + 2 * fi) * a.S2 * HP;
-        float2* dstB = a.X2 + ((long long)slot * a.np2 + 2 * fi + 1) * a.S2 * HP;
         store_band<M, RY, NW>(dstA, A, k0, nrows, lo, cnt);
-        store_band<M, RY, NW>(dstB, Bs, k0, nrows, lo, cnt);
-        __syncthreads();
+        store_band<M, RY, NW>(dstA + (long long)a.S2 * HP, Bs, k0, nrows, lo, cnt);
     }
 }
 
@@ -386,7 +473,8 @@ __global__ __launch_bounds__(NW * kWave) void k_p2_conv(EngineArgs a, int nf)
 // registers, inverse x-FFT, crop -> X2 plane planeOut.
 // ---------------------------------------------------------------------------
 template <int M, int RY, int NW, class PL>
-__global__ __launch_bounds__(NW * kWave) void k_p2_corr(EngineArgs a, int planeIn, int planeOut)
+__global__ __launch_bounds__(NW * kWave) void k_p2_corr(EngineArgs a, int planeIn, int planeOut,
+                                                        int tiles)
 {
     constexpr int V = M / kWave;
     constexpr int H = M / 2 + 1;
@@ -398,46 +486,59 @@ __global__ __launch_bounds__(NW * kWave) void k_p2_corr(EngineArgs a, int planeI
     const int lane = lane_id(), w = wave_id();
     PL FP;
     FP.load(a.tw, lane);
-    const P2Geom<M, RY> geo;
-    const int slot = geo.slot;
-    const Tile t = a.tiles[slot];
-    const int k0 = geo.band * RY;
-    const int nrows = geo.rows();
-    const int lo = t.ox0 - t.p0x - a.halo2;
-    const int cnt = t.ox1 - t.ox0 + 2 * a.halo2;
-    const float2* src = a.X1 + ((long long)slot * a.np1 + planeIn) * M * HP + (long long)k0 * M;
-    load_band<M, RY, NW>(A, src, nrows);
-    load_band<M, RY, NW>(Bs, src + (long long)M * HP, nrows);
-    __syncthreads();
-    for (int r = w; r < nrows; r += NW) {
-        float2* la = A + r * LP;
-        float2* lb = Bs + r * LP;
-        const float2* gxr = a.GT + (long long)(k0 + r) * M + lane;
-        const float2* gyr = a.GT + (long long)(H + k0 + r) * M + lane;
-        float2 gxv[V], gyv[V];
-#pragma unroll
-        for (int i = 0; i < V; ++i) {
-            gxv[i] = gxr[kWave * i];
-            gyv[i] = gyr[kWave * i];
+    const P2Items<M, RY, 1> items(tiles, 1);
+    auto band_src = [&](int slot, int band) {
+        return a.X1 + ((long long)slot * a.np1 + planeIn) * M * HP + (long long)band * RY * M;
+    };
+    // one item per block: a prefetch of two planes would cost the registers
+    // of the second block per CU
+    int slot, band, fi;
+    for (int k = 0; k < items.count; ++k) {
+        items.get(k, slot, band, fi);
+        const int k0 = band * RY;
+        const int nrows = items.rows(band);
+        {
+            BandRegs<M, RY, NW> pa, pb;
+            pa.fetch(band_src(slot, band), nrows);
+            pb.fetch(band_src(slot, band) + (long long)M * HP, nrows);
+            if (k) __syncthreads();  // the previous item's store has read A
+            pa.put(A);
+            pb.put(Bs);
         }
-        float2 X[V], Y[V];
-        fft_lds_to_regs<M, -1>(FP, la, X, lane);
-        fft_lds_to_regs<M, -1>(FP, lb, Y, lane);
+        __syncthreads();
+        for (int r = w; r < nrows; r += NW) {
+            float2* la = A + r * LP;
+            float2* lb = Bs + r * LP;
+            const float2* gxr = a.GT + (long long)(k0 + r) * M + lane;
+            const float2* gyr = a.GT + (long long)(H + k0 + r) * M + lane;
+            float2 gxv[V], gyv[V];
 #pragma unroll
-        for (int i = 0; i < V; ++i) X[i] = cadd(cmulc(X[i], gxv[i]), cmulc(Y[i], gyv[i]));
-        fft_from_regs<M, +1>(FP, la, X, lane);
+            for (int i = 0; i < V; ++i) {
+                gxv[i] = ldu(gxr, kWave * i);
+                gyv[i] = ldu(gyr, kWave * i);
+            }
+            float2 X[V], Y[V];
+            fft_lds_to_regs<M, -1>(FP, la, X, lane);
+            fft_lds_to_regs<M, -1>(FP, lb, Y, lane);
+#pragma unroll
+            for (int i = 0; i < V; ++i) X[i] = cadd(cmulc(X[i], gxv[i]), cmulc(Y[i], gyv[i]));
+            fft_from_regs<M, +1>(FP, la, X, lane);
+        }
+        __syncthreads();
+        const Tile t = a.tiles[slot];
+        const int lo = t.ox0 - t.p0x - a.halo2;
+        const int cnt = t.ox1 - t.ox0 + 2 * a.halo2;
+        store_band<M, RY, NW>(a.X2 + ((long long)slot * a.np2 + planeOut) * a.S2 * HP, A, k0,
+                              nrows, lo, cnt);
     }
-    __syncthreads();
-    store_band<M, RY, NW>(a.X2 + ((long long)slot * a.np2 + planeOut) * a.S2 * HP, A, k0, nrows,
-                          lo, cnt);
 }
 
 // ---------------------------------------------------------------------------
 // P3 forward / tangent: inverse y-FFT of (u_x + i u_y) per output column,
 // gradient stencil and combine, valid-region store.
 // ---------------------------------------------------------------------------
-template <int M, class PL>
-__global__ __launch_bounds__(256) void k_p3_fwd(EngineArgs a, int nf)
+template <int M, class PL, int NF>
+__global__ __launch_bounds__(256) void k_p3_fwd(EngineArgs a)
 {
     constexpr int V = M / kWave;
     constexpr int HP = spec_pitch(M);
@@ -452,44 +553,49 @@ __global__ __launch_bounds__(256) void k_p3_fwd(EngineArgs a, int nf)
     const int N = a.N, e = a.e, W = a.W;
     const int gx = t.ox0 + xi;
     float2* line = lds[w];
-    const float2* base = a.X2 + (long long)slot * a.np2 * a.S2 * HP + (long long)xi * HP;
     const long long pstride = (long long)a.S2 * HP;
-    const float* f0 = a.f0 + (long long)t.b * a.img_stride;
-    float* out = a.out0 + (long long)t.b * a.w_stride + (long long)(gx - e) * W;
+    const float2* base = a.X2 + (long long)slot * a.np2 * pstride + (long long)xi * HP;
+    // output rows [oy0, oy1) and the column lie inside the valid region, so
+    // every gradient stencil is inside the image
+    const Col3 F0 = col3(a.f0 + (long long)t.b * a.img_stride, N, gx);
+    const long long ob = (long long)t.b * a.w_stride + (long long)(gx - e) * W - e;
+    float* out = a.out0 + ob;  // indexed by the global row
 
     // u(g) for this column at tile rows y = lane + 64 j
     float2 u0[V];
     ifft_pair<M, PL>(FP, line, base, base + pstride, u0, lane);
-    if (nf == 1) {
+    if constexpr (NF == 1) {
 #pragma unroll
         for (int j = 0; j < V; ++j) {
             const int gy = t.p0y + lane + kWave * j;
-            if (gy >= t.oy0 && gy < t.oy1)
-                out[gy - e] = u0[j].x * grad_x(f0, N, gy, gx) + u0[j].y * grad_y(f0, N, gy, gx);
+            if (gy >= t.oy0 && gy < t.oy1) {  // conditional: keeps the loads late
+                const float2 g0 = grad_in(F0, (unsigned)gy);
+                stu(out, (unsigned)gy, u0[j].x * g0.x + u0[j].y * g0.y);
+            }
         }
         if (a.ux_out) {
-            const long long o = (long long)t.b * a.w_stride + (long long)(gx - e) * W - e;
 #pragma unroll
             for (int j = 0; j < V; ++j) {
                 const int gy = t.p0y + lane + kWave * j;
                 if (gy >= t.oy0 && gy < t.oy1) {
-                    a.ux_out[o + gy] = u0[j].x;
-                    a.uy_out[o + gy] = u0[j].y;
+                    stu(a.ux_out + ob, (unsigned)gy, u0[j].x);
+                    stu(a.uy_out + ob, (unsigned)gy, u0[j].y);
                 }
             }
         }
-        return;
-    }
-    // tangent: + u(dg) . grad f + u(g) . grad df
-    const float* f1 = a.f1 + (long long)t.b * a.img_stride;
-    float2 u1[V];
-    ifft_pair<M, PL>(FP, line, base + 2 * pstride, base + 3 * pstride, u1, lane);
+    } else {
+        // tangent: + u(dg) . grad f + u(g) . grad df
+        const Col3 F1 = col3(a.f1 + (long long)t.b * a.img_stride, N, gx);
+        float2 u1[V];
+        ifft_pair<M, PL>(FP, line, base + 2 * pstride, base + 3 * pstride, u1, lane);
 #pragma unroll
-    for (int j = 0; j < V; ++j) {
-        const int gy = t.p0y + lane + kWave * j;
-        if (gy >= t.oy0 && gy < t.oy1) {
-            out[gy - e] = u1[j].x * grad_x(f0, N, gy, gx) + u1[j].y * grad_y(f0, N, gy, gx) +
-                          u0[j].x * grad_x(f1, N, gy, gx) + u0[j].y * grad_y(f1, N, gy, gx);
+        for (int j = 0; j < V; ++j) {
+            const int gy = t.p0y + lane + kWave * j;
+            if (gy >= t.oy0 && gy < t.oy1) {
+                const float2 g0 = grad_in(F0, (unsigned)gy), g1 = grad_in(F1, (unsigned)gy);
+                stu(out, (unsigned)gy,
+                    u1[j].x * g0.x + u1[j].y * g0.y + u0[j].x * g1.x + u0[j].y * g1.y);
+            }
         }
     }
 }
@@ -533,10 +639,11 @@ __global__ __launch_bounds__(P3A_WAVES * kWave) void k_p3_adj(EngineArgs a, int 
     const long long pstride = (long long)a.S2 * HP;
     const float2* tbase = a.X2 + (long long)slot * a.np2 * pstride;
     const int ncols = cnt + 2;
-    // X2 column spectrum of output column j (tile-local), or null beyond the stored ones
+    // X2 column spectrum of output column j (tile-local), or the zero column
+    // beyond the stored ones
     auto col = [&](int plane, int j) -> const float2* {
         const int xi2 = j + 1;
-        return xi2 < ncols ? tbase + plane * pstride + (long long)xi2 * HP : nullptr;
+        return xi2 < ncols ? tbase + plane * pstride + (long long)xi2 * HP : a.zcol;
     };
     const long long ib = (long long)t.b * a.img_stride;
 
@@ -554,7 +661,8 @@ __global__ __launch_bounds__(P3A_WAVES * kWave) void k_p3_adj(EngineArgs a, int 
 #pragma unroll
                 for (int j = 0; j < V; ++j) {
                     const int gy = t.p0y + lane + kWave * j;
-                    cv[j] = (colv && gy >= e && gy < N - e) ? vcol[gy] : 0.f;
+                    const bool ok = colv && gy >= e && gy < N - e;
+                    cv[j] = ok ? vcol[gy] : 0.f;  // conditional: keeps the loads late
                 }
             };
             const int px = 2 * fi, py = 2 * fi + 1;
@@ -613,8 +721,8 @@ __global__ __launch_bounds__(P3A_WAVES * kWave) void k_p3_adj(EngineArgs a, int 
             for (int j = 0; j < V; ++j) {
                 const int gy = t.p0y + lane + kWave * j;
                 if (gy >= t.oy0 && gy < t.oy1) {
-                    o0[gy] = acc0[j];
-                    if (second) o0[N + gy] = acc1[j];
+                    stu(o0, (unsigned)gy, acc0[j]);
+                    if (second) stu(o0 + N, (unsigned)gy, acc1[j]);
                 }
             }
         }
@@ -629,8 +737,8 @@ __global__ __launch_bounds__(P3A_WAVES * kWave) void k_p3_adj(EngineArgs a, int 
         for (int j = 0; j < V; ++j) {
             const int gy = t.p0y + lane + kWave * j;
             if (gy >= t.oy0 && gy < t.oy1) {
-                o1[gy] = l[j].x;
-                if (second) o1[N + gy] = l[j].y;
+                stu(o1, (unsigned)gy, l[j].x);
+                if (second) stu(o1 + N, (unsigned)gy, l[j].y);
             }
         }
     }
@@ -699,11 +807,6 @@ struct P2Cfg<1024> { static constexpr int RY = 8, NW = 8; };
 template <>
 struct P2Cfg<2048> { static constexpr int RY = 4, NW = 4; };
 
-template <class PL>
-struct IsMfmaPlan { static constexpr bool value = false; };
-template <int M>
-struct IsMfmaPlan<FftPlanMfma<M>> { static constexpr bool value = true; };
-
 // Wavefronts per P2 block: one LDS line per wavefront, except that the MFMA
 // plan holds more per-lane constants and runs two lines per wavefront at
 // M >= 512 so that a lane may use 256 registers.
@@ -748,7 +851,7 @@ int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s, 
         EngineArgs a = a0;
         a.tiles = a0.tiles + t0;
         a.ry_log2 = RY == 16 ? 4 : RY == 8 ? 3 : 2;
-        const dim3 g1(M / 8, c), g1a(M / 4, c), g2(P2Geom<M, RY>::blocks(c));
+        const dim3 g1(M / 8, c), g1a(M / 4, c);
         const dim3 g3f((cols_out + 3) / 4, c), g3a((cols_out + 2 * P3A_WAVES - 1) / (2 * P3A_WAVES), c);
         const dim3 b256(256), b2(NW * kWave), b3a(P3A_WAVES * kWave);
         // launch one pass, bracketed by timing events when a timer is given
@@ -766,12 +869,21 @@ int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s, 
             run(kP1AdjSrc, [&] { k_p1_adj_src<M, PL><<<g1a, b256, 0, s>>>(a, nf, plane0); });
         };
         auto p2c = [&](int nf) {
-            run(kP2Conv, [&] { k_p2_conv<M, RY, NW, PL><<<g2, b2, lds2, s>>>(a, nf); });
+            const dim3 g2(P2Items<M, RY, P2C_ITEMS>::blocks(c, nf));
+            run(kP2Conv, [&] { k_p2_conv<M, RY, NW, PL><<<g2, b2, lds2, s>>>(a, nf, c); });
         };
         auto p2r = [&](int in, int out) {
-            run(kP2Corr, [&] { k_p2_corr<M, RY, NW, PL><<<g2, b2, lds2, s>>>(a, in, out); });
+            const dim3 g2(P2Items<M, RY, 1>::blocks(c, 1));
+            run(kP2Corr, [&] { k_p2_corr<M, RY, NW, PL><<<g2, b2, lds2, s>>>(a, in, out, c); });
         };
-        auto p3f = [&](int nf) { run(kP3Fwd, [&] { k_p3_fwd<M, PL><<<g3f, b256, 0, s>>>(a, nf); }); };
+        auto p3f = [&](int nf) {
+            run(kP3Fwd, [&] {
+                if (nf == 1)
+                    k_p3_fwd<M, PL, 1><<<g3f, b256, 0, s>>>(a);
+                else
+                    k_p3_fwd<M, PL, 2><<<g3f, b256, 0, s>>>(a);
+            });
+        };
         auto p3a = [&](int nf, int with_u, int planeL) {
             run(kP3Adj, [&] { k_p3_adj<M, PL><<<g3a, b3a, 0, s>>>(a, nf, with_u, planeL); });
         };

@@ -51,6 +51,7 @@ struct EngineArgs {
     int N, e, W;
     const float2* GT;      // [2][M/2+1][M] scaled spectra (Gx then Gy)
     const float2* tw;      // twiddles exp(-2 pi i k / M), k < M
+    const float2* zcol;    // one all-zero column spectrum (spec_pitch(M) elements)
     const Tile* tiles;     // tiles of this chunk
     float2* X1;            // [chunk][np1] planes of M * HP elements, band layout (x1_index)
     float2* X2;            // [chunk][np2][S2][HP]

@@ -98,3 +98,14 @@ def test_kernel_kind_names_and_timing_null_workspace():
     assert L.srkKernelKindName(6) == b"" and L.srkKernelKindName(-1) == b""
     assert L.srkWorkspaceSetTiming(None, 1) == -3
     assert L.srkWorkspaceKernelTimes(None, None, None, None) == -3
+
+
+def test_cd_entry_points_reject_bad_arguments():
+    L = _lib.lib()
+    # null pointers, bad batch, bad sizes: rejected before any GPU work
+    assert L.srkMeasureCDBatched(None, 1, 64, 16, 8, 0.0, None, None, None, None) == -3
+    p = 256  # any non-null value: the checks below fail first
+    assert L.srkMeasureCDBatched(p, 0, 64, 16, 8, 0.0, p, p, p, None) == -7
+    assert L.srkMeasureCDBatched(p, 1, 1, 16, 8, 0.0, p, p, p, None) == -2
+    assert L.srkMeasureCDBatched(p, 1, 50000, 16, 8, 0.0, p, p, p, None) == -5
+    assert L.srkShrinkageCDBatched(p, p, 256, 100, 60, 1, 1.0, 1, 16, 8, 0.0, p, p, p, None) == -2
