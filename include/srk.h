@@ -89,6 +89,20 @@ int srkPrecomputeKernelFreq(const float* h_G, int kw, int kh, int M, void* d_G_f
 int srkKernelSpectraSRSRK(int kw, double sigma, double dx, double cw, int M, void* d_Gx_freq,
                           void* d_Gy_freq, float* h_Gx, float* h_Gy, hipStream_t stream);
 
+/*
+ * Device counterpart of srkKernelSpectraSRSRK (replaces the host round trips of
+ * srkPrecomputeKernelFreq, srk_forward.cuh:154-203): the kernel values (FP64
+ * K1, Blackman window) and the separable 2-D DFT are all evaluated in FP64
+ * on the device; only the spectra (and the optional spatial kernels d_Kx /
+ * d_Ky, kw*kw floats, col-major, device) are rounded to FP32.  Asynchronous
+ * on the stream, no host transfers; takes stream-ordered scratch
+ * (hipMallocAsync / hipFreeAsync) of 2 kw^2 + 2 M + 4 kw (M/2+1) doubles.
+ * Same arguments and errors as srkKernelSpectraSRSRK.
+ */
+int srkKernelSpectraSRSRKDevice(int kw, double sigma, double dx, double cw, int M,
+                                void* d_Gx_freq, void* d_Gy_freq, float* d_Kx, float* d_Ky,
+                                hipStream_t stream);
+
 /* ---- workspace ---------------------------------------------------------- */
 
 /*

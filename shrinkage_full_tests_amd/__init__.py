@@ -15,7 +15,8 @@ import ctypes
 from ._lib import SrkError, call, lib  # noqa: F401
 
 __all__ = [
-    "SrkError", "lib", "Workspace", "kernel_spectra_srsrk", "precompute_kernel_freq",
+    "SrkError", "lib", "Workspace", "kernel_spectra_srsrk", "kernel_spectra_srsrk_device",
+    "precompute_kernel_freq",
     "srkTrainingForward", "srkForwardOaS", "adjointSRK", "shrinkageTangentOaS",
     "shrinkageTangentAdjointOaS", "forward_batched", "adjoint_batched", "tangent_batched",
     "tangent_adjoint_batched", "ShrinkageFunction", "erosion_for", "SUPPORTED_M",
@@ -137,6 +138,22 @@ def kernel_spectra_srsrk(kw: int, sigma: float, M: int, dx: float = 1.0, cw: flo
     if with_spatial:
         return gx, gy, hx, hy
     return gx, gy
+
+
+def kernel_spectra_srsrk_device(kw: int, sigma: float, M: int, dx: float = 1.0, cw: float = 3.0,
+                                with_spatial: bool = False, stream=None):
+    """SRSRK spectra computed in FP64 on the device (no host round trip);
+    with_spatial also returns the [kw, kw] spatial kernels as device tensors."""
+    torch = _torch()
+    gx = torch.empty((M, M // 2 + 1), dtype=torch.complex64, device="cuda")
+    gy = torch.empty_like(gx)
+    hx = hy = None
+    if with_spatial:
+        hx = torch.empty((kw, kw), dtype=torch.float32, device="cuda")
+        hy = torch.empty_like(hx)
+    call("srkKernelSpectraSRSRKDevice", kw, float(sigma), float(dx), float(cw), M, _ptr(gx),
+         _ptr(gy), _ptr(hx), _ptr(hy), _stream(stream))
+    return (gx, gy, hx, hy) if with_spatial else (gx, gy)
 
 
 def precompute_kernel_freq(G, M: int, device="cuda"):

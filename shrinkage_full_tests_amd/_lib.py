@@ -23,6 +23,7 @@ SIGNATURES = {
     "srkSynchronize": (_i, [_p]),
     "srkPrecomputeKernelFreq": (_i, [_p, _i, _i, _i, _p, _p]),
     "srkKernelSpectraSRSRK": (_i, [_i, _d, _d, _d, _i, _p, _p, _p, _p, _p]),
+    "srkKernelSpectraSRSRKDevice": (_i, [_i, _d, _d, _d, _i, _p, _p, _p, _p, _p]),
     "srkWorkspaceCreate": (_i, [_i, _i, _i, _i, _i, _sz, ctypes.POINTER(_p)]),
     "srkWorkspaceDestroy": (_i, [_p]),
     "srkWorkspaceBytes": (_sz, [_p]),

@@ -4,6 +4,7 @@
 
 #include "srk_engine.hpp"
 #include "srk_cd.hpp"
+#include "srk_spectra.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -461,6 +462,24 @@ int srkKernelSpectraSRSRK(int kw, double sigma, double dx, double cw, int M, voi
                                           hipMemcpyHostToDevice, stream)));
     if (!d_Gx_freq && !d_Gy_freq) return SRK_OK;  // host kernels only: no device work
     return hip_status(hipStreamSynchronize(stream));
+}
+
+int srkKernelSpectraSRSRKDevice(int kw, double sigma, double dx, double cw, int M,
+                                void* d_Gx_freq, void* d_Gy_freq, float* d_Kx, float* d_Ky,
+                                hipStream_t stream)
+{
+    if (kw < 3 || kw % 2 == 0 || !(sigma > 0) || !(dx > 0) || !(cw > 0)) return SRK_ERR_KERNEL;
+    if (M < 2 || (M & (M - 1)) || kw > M) return SRK_ERR_UNSUPPORTED_SIZE;
+    if (!d_Gx_freq || !d_Gy_freq) return SRK_ERR_NULL_PTR;
+    if (!aligned((const char*)d_Gx_freq, 8) || !aligned((const char*)d_Gy_freq, 8))
+        return SRK_ERR_MISALIGNED;
+    void* scratch = nullptr;
+    SRK_TRY(hip_status(hipMallocAsync(&scratch, srk::spectra_scratch_bytes(kw, M), stream)));
+    const int st = hip_status(srk::spectra_srsrk_device(kw, sigma, dx, cw, M, scratch,
+                                                        (float2*)d_Gx_freq, (float2*)d_Gy_freq,
+                                                        d_Kx, d_Ky, stream));
+    const int st2 = hip_status(hipFreeAsync(scratch, stream));
+    return st != SRK_OK ? st : st2;
 }
 
 int srkWorkspaceCreate(int N, int W, int erosion, int M, int B_max, size_t scratch_mb,
