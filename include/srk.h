@@ -228,6 +228,20 @@ int srkTangentAdjointBatched(const float* f, const float* g, const float* df, co
                              hipStream_t stream, float* dLam_f, float* dLam_g,
                              srk_workspace_t ws);
 
+/* ---- reference test inputs (host) ---------------------------------------- */
+
+/*
+ * Replaces GpGenerator (third_party/include/stTestCore/gp_generator.h:95-201)
+ * as used by TestPatternConfig::gaussian_process (third_party/src/patternUtil/
+ * test_pattern.cu:156-160): count consecutive Gaussian-process fields of W x H
+ * (column-major floats, written to the host array h_out [count][W*H]) with
+ * covariance sigma^2 exp(-Q/2) of length scales ell_x, ell_y and correlation
+ * rho, drawn from one std::mt19937_64 stream seeded with seed (the reference
+ * replaces seed 0 by a random seed; here 0 is used as given).  Host only.
+ */
+int srkGpGenerate(double sigma, double ell_x, double ell_y, double rho, int W, int H,
+                  unsigned long long seed, int count, float* h_out);
+
 /* ---- critical-dimension (CD) measurement -------------------------------- */
 
 /*

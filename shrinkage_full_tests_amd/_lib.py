@@ -46,6 +46,7 @@ SIGNATURES = {
     "srkTangentAdjointBatched": (_i, [_p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _f, _i, _p,
                                       _p, _p, _p]),
     "srkMeasureCDBatched": (_i, [_p, _i, _i, _i, _i, _f, _p, _p, _p, _p]),
+    "srkGpGenerate": (_i, [_d, _d, _d, _d, _i, _i, ctypes.c_ulonglong, _i, _p]),
     "srkShrinkageCDBatched": (_i, [_p, _p, _i, _i, _i, _i, _f, _i, _i, _i, _f, _p, _p, _p, _p]),
 }
 

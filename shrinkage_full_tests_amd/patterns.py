@@ -9,6 +9,11 @@
   Gaussian smoothing of width sigma, standardised, then sigmoid to (0, 1).
 - ``lns_box``: line/space of pitch 64 px, duty 0.5, box edges blurred by a
   Gaussian of sigma 2 px, values in [0, 1].
+- ``gp_field`` / ``gp_fields``: the reference's Gaussian-process fixtures
+  (GpGenerator, third_party/include/stTestCore/gp_generator.h:95-201), e.g.
+  ``gp_field(N, 32.0, 42)`` / ``(N, 32.0, 43)`` for the f / g inputs of
+  tan_tanadj/main_tan_tanadj.cu:322-323; generated by srkGpGenerate in the
+  native library (host code, same mt19937_64 / normal_distribution stream).
 """
 from __future__ import annotations
 
@@ -87,3 +92,20 @@ def grf_torch(B: int, N: int, seed: int, sigma: float = 8.0, device="cuda"):
         zs = zs / zs.std(dim=(1, 2), keepdim=True)
         out[b0:b1] = torch.sigmoid(zs)
     return out
+
+
+def gp_fields(W: int, H: int, ell_x: float, seed: int, count: int = 1, ell_y: float = 0.0,
+              rho: float = 0.0, sigma: float = 1.0) -> np.ndarray:
+    """[count, W, H] float32 consecutive GP fields of one seeded stream
+    (column-major: [x, y] indexing), covariance sigma^2 exp(-Q/2)."""
+    import ctypes
+    from ._lib import call
+    out = np.empty((count, W, H), np.float32)
+    call("srkGpGenerate", float(sigma), float(ell_x), float(ell_y or ell_x), float(rho), W, H,
+         ctypes.c_ulonglong(seed), count, out.ctypes.data)
+    return out
+
+
+def gp_field(N: int, length_scale_px: float = 32.0, seed: int = 42) -> np.ndarray:
+    """TestPatternConfig::gaussian_process(N, length_scale_px, seed): [N, N] float32."""
+    return gp_fields(N, N, length_scale_px, seed)[0]
