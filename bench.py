@@ -90,6 +90,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=0, help="tiles per chunk (0 = auto)")
     ap.add_argument("--p2-rows", type=int, default=0, help="k_y bands per row-pass block (0 = auto)")
     ap.add_argument("--scratch-mb", type=int, default=0)
+    ap.add_argument("--concurrency", type=int, default=-1,
+                    help="adjoint front passes on two streams (1) or one (0); -1 = library default")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--precision", default="both", choices=("fp32", "mixed_fp16", "both"))
     ap.add_argument("--no-cpu", action="store_true")
@@ -136,7 +138,8 @@ def main():
     lf = torch.empty((B, N, N), device="cuda")
     lg = torch.empty((B, N, N), device="cuda")
     ws = srk.Workspace(N, W, e, M, B_max=B, scratch_mb=args.scratch_mb)
-    ws.set_tuning(chunk_tiles=args.chunk, p2_rows=args.p2_rows)
+    ws.set_tuning(chunk_tiles=args.chunk, p2_rows=args.p2_rows,
+                  concurrency=None if args.concurrency < 0 else args.concurrency)
     stream = torch.cuda.current_stream()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 

@@ -88,12 +88,15 @@ class Workspace:
     def set_chunk(self, tiles: int) -> None:
         call("srkWorkspaceSetChunk", self.handle, int(tiles))
 
-    def set_tuning(self, chunk_tiles=None, p2_rows=None) -> None:
-        """Autotuner knobs: tiles per launch chunk, k_y bands per row-pass block."""
+    def set_tuning(self, chunk_tiles=None, p2_rows=None, concurrency=None) -> None:
+        """Autotuner knobs: tiles per launch chunk, k_y bands per row-pass block,
+        two-stream adjoint front passes (1, the default) or one stream (0)."""
         if chunk_tiles is not None:
             call("srkWorkspaceSetTuning", self.handle, 0, int(chunk_tiles))
         if p2_rows is not None:
             call("srkWorkspaceSetTuning", self.handle, 1, int(p2_rows))
+        if concurrency is not None:
+            call("srkWorkspaceSetTuning", self.handle, 2, int(concurrency))
 
     def set_precision(self, precision) -> None:
         """"fp32" (default) or "mixed_fp16" (FP16 MFMA DFT stages, FP32 accumulation)."""

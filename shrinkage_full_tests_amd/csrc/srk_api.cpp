@@ -257,6 +257,8 @@ struct srk_workspace_s {
     int precision = SRK_PREC_FP32;
     bool timing = false;
     srk::LaunchTimer timer;
+    bool concurrent = true;  // adjoint front passes on two streams
+    srk::Fork fork{nullptr, nullptr, nullptr};
     srk::Tile* d_ftiles = nullptr;
     srk::Tile* d_atiles = nullptr;
     int nft = 0, nat = 0, S2f = 0, S2a = 0;
@@ -273,6 +275,9 @@ namespace {
 void ws_free(srk_workspace_s* w)
 {
     if (!w) return;
+    if (w->fork.join) hipEventDestroy(w->fork.join);
+    if (w->fork.fork) hipEventDestroy(w->fork.fork);
+    if (w->fork.aux) hipStreamDestroy(w->fork.aux);
     hipFree(w->d_ftiles);
     hipFree(w->d_atiles);
     hipFree(w->X1);
@@ -385,8 +390,8 @@ int run_adj(srk_workspace_s* w, srk::Op op, srk::EngineArgs a, int B, hipStream_
     a.halo2 = 1;
     const int ntiles = B * w->nat;
     const int chunk = chunk_for(w, a.np1, a.np2, a.S2, ntiles);
-    return hip_status(
-        srk::engine_run(op, w->M, w->precision, a, ntiles, chunk, s, timer_for(w, s)));
+    return hip_status(srk::engine_run(op, w->M, w->precision, a, ntiles, chunk, s,
+                                      timer_for(w, s), w->concurrent ? &w->fork : nullptr));
 }
 
 }  // namespace
@@ -549,6 +554,11 @@ int srkWorkspaceCreate(int N, int W, int erosion, int M, int B_max, size_t scrat
     alloc((void**)&w->X2, w->x2_bytes);
     alloc((void**)&w->GT, (size_t)2 * (M / 2 + 1) * M * sizeof(float2));
     alloc((void**)&w->tw, tw.size() * sizeof(float2));
+    if (st == SRK_OK) st = hip_status(hipStreamCreateWithFlags(&w->fork.aux, hipStreamNonBlocking));
+    if (st == SRK_OK)
+        st = hip_status(hipEventCreateWithFlags(&w->fork.fork, hipEventDisableTiming));
+    if (st == SRK_OK)
+        st = hip_status(hipEventCreateWithFlags(&w->fork.join, hipEventDisableTiming));
     if (st == SRK_OK)
         st = hip_status(hipMemcpy(w->d_ftiles, fb.data(), fb.size() * sizeof(srk::Tile), hipMemcpyHostToDevice));
     if (st == SRK_OK)
@@ -593,6 +603,10 @@ int srkWorkspaceSetTuning(srk_workspace_t w, int key, int value)
     case SRK_TUNE_P2_ROWS:
         if (value != 0 && value != 4 && value != 8 && value != 16) return SRK_ERR_INVALID_ARG;
         w->p2_rows = value;
+        return SRK_OK;
+    case SRK_TUNE_CONCURRENCY:
+        if (value != 0 && value != 1) return SRK_ERR_INVALID_ARG;
+        w->concurrent = value != 0;
         return SRK_OK;
     default:
         return SRK_ERR_INVALID_ARG;

@@ -838,7 +838,8 @@ hipError_t prepare_kernels()
 }
 
 template <int M, class PL, int RY>
-int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s, LaunchTimer* tm)
+int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s, LaunchTimer* tm,
+             const Fork* fk)
 {
     static_assert(RY == 16 || RY == 8 || RY == 4, "band heights");
     constexpr int NW = P2Waves<M, RY, PL>::NW;
@@ -854,30 +855,32 @@ int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s, 
         const dim3 g1(M / 8, c), g1a(M / 4, c);
         const dim3 g3f((cols_out + 3) / 4, c), g3a((cols_out + 2 * P3A_WAVES - 1) / (2 * P3A_WAVES), c);
         const dim3 b256(256), b2(NW * kWave), b3a(P3A_WAVES * kWave);
-        // launch one pass, bracketed by timing events when a timer is given
-        auto run = [&](int kind, auto&& launch) {
+        // launch one pass on stream ls, bracketed by timing events when a timer is given
+        auto run = [&](int kind, hipStream_t ls, auto&& launch) {
             hipEvent_t e0 = nullptr, e1 = nullptr;
-            if (tm && st == hipSuccess) st = tm->record(s, &e0);
+            if (tm && st == hipSuccess) st = tm->record(ls, &e0);
             launch();
             if (tm && st == hipSuccess) {
-                st = tm->record(s, &e1);
+                st = tm->record(ls, &e1);
                 tm->recs.push_back({kind, c, e0, e1});
             }
         };
-        auto p1 = [&](int nf) { run(kP1Cols, [&] { k_p1_cols<M, PL><<<g1, b256, 0, s>>>(a, nf); }); };
-        auto p1a = [&](int nf, int plane0) {
-            run(kP1AdjSrc, [&] { k_p1_adj_src<M, PL><<<g1a, b256, 0, s>>>(a, nf, plane0); });
+        auto p1 = [&](int nf, hipStream_t ls) {
+            run(kP1Cols, ls, [&] { k_p1_cols<M, PL><<<g1, b256, 0, ls>>>(a, nf); });
         };
-        auto p2c = [&](int nf) {
+        auto p1a = [&](int nf, int plane0, hipStream_t ls) {
+            run(kP1AdjSrc, ls, [&] { k_p1_adj_src<M, PL><<<g1a, b256, 0, ls>>>(a, nf, plane0); });
+        };
+        auto p2c = [&](int nf, hipStream_t ls) {
             const dim3 g2(P2Items<M, RY, P2C_ITEMS>::blocks(c, nf));
-            run(kP2Conv, [&] { k_p2_conv<M, RY, NW, PL><<<g2, b2, lds2, s>>>(a, nf, c); });
+            run(kP2Conv, ls, [&] { k_p2_conv<M, RY, NW, PL><<<g2, b2, lds2, ls>>>(a, nf, c); });
         };
-        auto p2r = [&](int in, int out) {
+        auto p2r = [&](int in, int out, hipStream_t ls) {
             const dim3 g2(P2Items<M, RY, 1>::blocks(c, 1));
-            run(kP2Corr, [&] { k_p2_corr<M, RY, NW, PL><<<g2, b2, lds2, s>>>(a, in, out, c); });
+            run(kP2Corr, ls, [&] { k_p2_corr<M, RY, NW, PL><<<g2, b2, lds2, ls>>>(a, in, out, c); });
         };
         auto p3f = [&](int nf) {
-            run(kP3Fwd, [&] {
+            run(kP3Fwd, s, [&] {
                 if (nf == 1)
                     k_p3_fwd<M, PL, 1><<<g3f, b256, 0, s>>>(a);
                 else
@@ -885,36 +888,48 @@ int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s, 
             });
         };
         auto p3a = [&](int nf, int with_u, int planeL) {
-            run(kP3Adj, [&] { k_p3_adj<M, PL><<<g3a, b3a, 0, s>>>(a, nf, with_u, planeL); });
+            run(kP3Adj, s, [&] { k_p3_adj<M, PL><<<g3a, b3a, 0, s>>>(a, nf, with_u, planeL); });
+        };
+        // adjoint operators: convolved fields (P1 + P2 conv) and adjoint sources
+        // (P1A + P2 corr) are independent until P3 and may run concurrently
+        auto adjoint_front = [&](int nf, int plane_src, int plane_corr) {
+            hipStream_t sa = s;
+            if (fk && st == hipSuccess) {
+                st = hipEventRecord(fk->fork, s);
+                if (st == hipSuccess) st = hipStreamWaitEvent(fk->aux, fk->fork, 0);
+                sa = fk->aux;
+            }
+            p1(nf, sa);
+            p2c(nf, sa);
+            p1a(nf, plane_src, s);
+            p2r(plane_src, plane_corr, s);
+            if (fk && st == hipSuccess) {
+                st = hipEventRecord(fk->join, fk->aux);
+                if (st == hipSuccess) st = hipStreamWaitEvent(s, fk->join, 0);
+            }
         };
         switch (op) {
         case Op::Forward:
-            p1(1);
-            p2c(1);
+            p1(1, s);
+            p2c(1, s);
             p3f(1);
             break;
         case Op::Tangent:
-            p1(2);
-            p2c(2);
+            p1(2, s);
+            p2c(2, s);
             p3f(2);
             break;
         case Op::Adjoint:
-            p1(1);
-            p1a(1, 1);
-            p2c(1);
-            p2r(1, 2);
+            adjoint_front(1, 1, 2);
             p3a(1, 1, 2);
             break;
         case Op::AdjointG:
-            p1a(1, 0);
-            p2r(0, 0);
+            p1a(1, 0, s);
+            p2r(0, 0, s);
             p3a(1, 0, 0);
             break;
         case Op::TangentAdjoint:
-            p1(2);
-            p1a(2, 2);
-            p2c(2);
-            p2r(2, 4);
+            adjoint_front(2, 2, 4);
             p3a(2, 1, 4);
             break;
         }
@@ -929,37 +944,37 @@ int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s, 
 // otherwise the default for M.
 template <int M, class PL>
 int run_rows(Op op, const EngineArgs& a, int ntiles, int chunk, int rows, hipStream_t s,
-             LaunchTimer* tm)
+             LaunchTimer* tm, const Fork* fk)
 {
     if constexpr (M <= 512) {
-        if (rows == 16) return run_impl<M, PL, 16>(op, a, ntiles, chunk, s, tm);
-        if (rows == 4) return run_impl<M, PL, 4>(op, a, ntiles, chunk, s, tm);
+        if (rows == 16) return run_impl<M, PL, 16>(op, a, ntiles, chunk, s, tm, fk);
+        if (rows == 4) return run_impl<M, PL, 4>(op, a, ntiles, chunk, s, tm, fk);
     }
-    return run_impl<M, PL, P2Cfg<M>::RY>(op, a, ntiles, chunk, s, tm);
+    return run_impl<M, PL, P2Cfg<M>::RY>(op, a, ntiles, chunk, s, tm, fk);
 }
 
 }  // namespace
 
 int engine_run(Op op, int M, int prec, const EngineArgs& a, int ntiles, int chunk, hipStream_t s,
-               LaunchTimer* tm)
+               LaunchTimer* tm, const Fork* fk)
 {
     if (chunk < 1 || ntiles < 0) return (int)hipErrorInvalidValue;
     if (ntiles == 0) return 0;
     const int rows = a.p2_rows;
     if (prec == 0) {
         switch (M) {
-        case 256: return run_rows<256, FftPlan<256>>(op, a, ntiles, chunk, rows, s, tm);
-        case 512: return run_rows<512, FftPlan<512>>(op, a, ntiles, chunk, rows, s, tm);
-        case 1024: return run_rows<1024, FftPlan<1024>>(op, a, ntiles, chunk, rows, s, tm);
-        case 2048: return run_rows<2048, FftPlan<2048>>(op, a, ntiles, chunk, rows, s, tm);
+        case 256: return run_rows<256, FftPlan<256>>(op, a, ntiles, chunk, rows, s, tm, fk);
+        case 512: return run_rows<512, FftPlan<512>>(op, a, ntiles, chunk, rows, s, tm, fk);
+        case 1024: return run_rows<1024, FftPlan<1024>>(op, a, ntiles, chunk, rows, s, tm, fk);
+        case 2048: return run_rows<2048, FftPlan<2048>>(op, a, ntiles, chunk, rows, s, tm, fk);
         default: return (int)hipErrorInvalidValue;
         }
     }
     if (prec == 1) {
         switch (M) {
-        case 256: return run_rows<256, FftPlanMfma<256>>(op, a, ntiles, chunk, rows, s, tm);
-        case 512: return run_rows<512, FftPlanMfma<512>>(op, a, ntiles, chunk, rows, s, tm);
-        case 1024: return run_rows<1024, FftPlanMfma<1024>>(op, a, ntiles, chunk, rows, s, tm);
+        case 256: return run_rows<256, FftPlanMfma<256>>(op, a, ntiles, chunk, rows, s, tm, fk);
+        case 512: return run_rows<512, FftPlanMfma<512>>(op, a, ntiles, chunk, rows, s, tm, fk);
+        case 1024: return run_rows<1024, FftPlanMfma<1024>>(op, a, ntiles, chunk, rows, s, tm, fk);
         default: return (int)hipErrorInvalidValue;
         }
     }

@@ -113,11 +113,22 @@ struct LaunchTimer {
     ~LaunchTimer();
 };
 
+// Second stream for the independent halves of the adjoint passes: the
+// convolved field (P1, P2 conv) runs on `aux` while the adjoint sources
+// (P1A, P2 corr) run on the caller's stream; `fork` / `join` order them
+// (event record + stream wait, so the pattern can be captured in a graph).
+struct Fork {
+    hipStream_t aux;
+    hipEvent_t fork, join;
+};
+
 // Launch the passes of `op` for tiles [0, ntiles) of the tile table
 // args.tiles, chunk tiles at a time.  prec 0: FP32 transforms; 1: FP16 MFMA
-// DFT stages with FP32 accumulation (M <= 1024).  Returns a hipError_t.
+// DFT stages with FP32 accumulation (M <= 1024).  With `fork`, the adjoint
+// operators split their first two passes over two streams.  Returns a
+// hipError_t.
 int engine_run(Op op, int M, int prec, const EngineArgs& args, int ntiles, int chunk,
-               hipStream_t stream, LaunchTimer* timer = nullptr);
+               hipStream_t stream, LaunchTimer* timer = nullptr, const Fork* fork = nullptr);
 
 // Transpose + scale reference-layout spectra (Ghat[k_x*(M/2+1)+k_y]) into GT.
 int engine_import_spectra(int M, const float2* Gx_ref, const float2* Gy_ref, float scale,
