@@ -615,10 +615,14 @@ __global__ __launch_bounds__(256) void k_p3_fwd(EngineArgs a)
 // ---------------------------------------------------------------------------
 constexpr int P3A_WAVES = 4;
 
-template <int M, class PL>
-__global__ __launch_bounds__(P3A_WAVES * kWave) void k_p3_adj(EngineArgs a, int nf, int with_u,
-                                                              int planeL)
+// One instantiation per field count (a runtime loop over the fields keeps
+// both iterations' state live); the FP32 single-field form at M <= 512 fits
+// 128 registers, four wavefronts per SIMD.
+template <int M, class PL, int NF>
+__global__ __launch_bounds__(P3A_WAVES * kWave)
+__attribute__((amdgpu_waves_per_eu(M <= 512 && NF == 1 && !IsMfmaPlan<PL>::value ? 4 : 1))) void k_p3_adj(EngineArgs a, int with_u, int planeL)
 {
+    constexpr int nf = NF;
     constexpr int V = M / kWave;
     constexpr int HP = spec_pitch(M);
     constexpr int CBK = 2 * P3A_WAVES;  // output columns per block
@@ -888,7 +892,12 @@ int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s, 
             });
         };
         auto p3a = [&](int nf, int with_u, int planeL) {
-            run(kP3Adj, s, [&] { k_p3_adj<M, PL><<<g3a, b3a, 0, s>>>(a, nf, with_u, planeL); });
+            run(kP3Adj, s, [&] {
+                if (nf == 1)
+                    k_p3_adj<M, PL, 1><<<g3a, b3a, 0, s>>>(a, with_u, planeL);
+                else
+                    k_p3_adj<M, PL, 2><<<g3a, b3a, 0, s>>>(a, with_u, planeL);
+            });
         };
         // adjoint operators: convolved fields (P1 + P2 conv) and adjoint sources
         // (P1A + P2 corr) are independent until P3 and may run concurrently
