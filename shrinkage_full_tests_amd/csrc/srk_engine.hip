@@ -180,9 +180,10 @@ __device__ __forceinline__ float2 grad_in(const Col3& f, unsigned yy)
 // P1: real FFT along y of the convolved fields; one wavefront per column pair,
 // 8 columns per block.
 // ---------------------------------------------------------------------------
-template <int M, class PL>
-__global__ __launch_bounds__(256) void k_p1_cols(EngineArgs a, int nf)
+template <int M, class PL, int NF>
+__global__ __launch_bounds__(256) void k_p1_cols(EngineArgs a)
 {
+    constexpr int nf = NF;
     constexpr int V = M / kWave;
     constexpr int HP = spec_pitch(M);
     __shared__ float2 lds[4][line_slots(M)];
@@ -240,9 +241,10 @@ __global__ __launch_bounds__(256) void k_p1_cols(EngineArgs a, int nf)
 // loads need no bounds checks: rows outside it load at a clamped row and are
 // masked.
 // ---------------------------------------------------------------------------
-template <int M, class PL>
-__global__ __launch_bounds__(256) void k_p1_adj_src(EngineArgs a, int nf, int plane0)
+template <int M, class PL, int NF>
+__global__ __launch_bounds__(256) void k_p1_adj_src(EngineArgs a, int plane0)
 {
+    constexpr int nf = NF;
     constexpr int V = M / kWave;
     constexpr int HP = spec_pitch(M);
     __shared__ float2 lds[4][line_slots(M)];
@@ -615,9 +617,6 @@ __global__ __launch_bounds__(256) void k_p3_fwd(EngineArgs a)
 // ---------------------------------------------------------------------------
 constexpr int P3A_WAVES = 4;
 
-// One instantiation per field count (a runtime loop over the fields keeps
-// both iterations' state live); the FP32 single-field form at M <= 512 fits
-// 128 registers, four wavefronts per SIMD.
 template <int M, class PL, int NF>
 __global__ __launch_bounds__(P3A_WAVES * kWave)
 __attribute__((amdgpu_waves_per_eu(M <= 512 && NF == 1 && !IsMfmaPlan<PL>::value ? 4 : 1))) void k_p3_adj(EngineArgs a, int with_u, int planeL)
@@ -870,10 +869,20 @@ int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s, 
             }
         };
         auto p1 = [&](int nf, hipStream_t ls) {
-            run(kP1Cols, ls, [&] { k_p1_cols<M, PL><<<g1, b256, 0, ls>>>(a, nf); });
+            run(kP1Cols, ls, [&] {
+                if (nf == 1)
+                    k_p1_cols<M, PL, 1><<<g1, b256, 0, ls>>>(a);
+                else
+                    k_p1_cols<M, PL, 2><<<g1, b256, 0, ls>>>(a);
+            });
         };
         auto p1a = [&](int nf, int plane0, hipStream_t ls) {
-            run(kP1AdjSrc, ls, [&] { k_p1_adj_src<M, PL><<<g1a, b256, 0, ls>>>(a, nf, plane0); });
+            run(kP1AdjSrc, ls, [&] {
+                if (nf == 1)
+                    k_p1_adj_src<M, PL, 1><<<g1a, b256, 0, ls>>>(a, plane0);
+                else
+                    k_p1_adj_src<M, PL, 2><<<g1a, b256, 0, ls>>>(a, plane0);
+            });
         };
         auto p2c = [&](int nf, hipStream_t ls) {
             const dim3 g2(P2Items<M, RY, P2C_ITEMS>::blocks(c, nf));
