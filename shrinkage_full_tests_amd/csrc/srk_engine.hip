@@ -178,16 +178,19 @@ __device__ __forceinline__ float2 grad_in(const Col3& f, unsigned yy)
 
 // ---------------------------------------------------------------------------
 // P1: real FFT along y of the convolved fields; one wavefront per column pair,
-// 8 columns per block.
+// 8 columns per block.  Each wavefront unpacks its pair's half spectra into
+// its own LDS line, then the block stores the 8 columns of every band as one
+// contiguous run of the X1 band layout.
 // ---------------------------------------------------------------------------
 template <int M, class PL, int NF>
 __global__ __launch_bounds__(256) void k_p1_cols(EngineArgs a)
 {
     constexpr int nf = NF;
     constexpr int V = M / kWave;
+    constexpr int H = M / 2 + 1;
     constexpr int HP = spec_pitch(M);
+    constexpr int KV = (H + kWave - 1) / kWave;  // half-spectrum values per lane
     __shared__ float2 lds[4][line_slots(M)];
-    __shared__ float2 st[stage_elems<M, 8>()];
     const int lane = lane_id(), w = wave_id();
     PL FP;
     FP.load(a.tw, lane);
@@ -199,10 +202,12 @@ __global__ __launch_bounds__(256) void k_p1_cols(EngineArgs a)
     const int ca = t.p0x + xa, cb = ca + 1;
     const bool va = ca >= 0 && ca < N, vb = cb >= 0 && cb < N;
     float2* line = lds[w];
+    const int RY = 1 << a.ry_log2;
     for (int fi = 0; fi < nf; ++fi) {
+        if (fi) __syncthreads();  // the previous field's stores have read the lines
+        // the wave's line ends up holding P (k <= M/2) at [k] and Q at [H + k]
         if (!va && !vb) {
-            zero_stage<M, 8>(st, 2 * w, lane);
-            zero_stage<M, 8>(st, 2 * w + 1, lane);
+            for (int k = lane; k < 2 * H; k += kWave) line[k] = make_float2(0.f, 0.f);
         } else {
             const float* src = (fi == 0 ? a.g0 : a.g1) + (long long)t.b * a.img_stride;
             // clamped addresses keep every load in bounds; masks zero the padding
@@ -226,11 +231,40 @@ __global__ __launch_bounds__(256) void k_p1_cols(EngineArgs a)
                 }
             }
             fft_from_regs<M, -1>(FP, line, v, lane);
-            unpack_stage<M, 8>(line, st, 2 * w, st, 2 * w + 1, lane);
+            // unpack Z(k), Z(M-k) into P, Q: all reads, then all writes (one wavefront,
+            // LDS operations in order)
+            float2 P[KV], Q[KV];
+#pragma unroll
+            for (int i = 0; i < KV; ++i) {
+                const int k = lane + kWave * i;
+                if (k < H) {
+                    const float2 z = line[lslot(k)];
+                    const float2 zm = line[lslot((M - k) & (M - 1))];
+                    P[i] = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
+                    Q[i] = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < KV; ++i) {
+                const int k = lane + kWave * i;
+                if (k < H) {
+                    line[k] = P[i];
+                    line[H + k] = Q[i];
+                }
+            }
         }
         __syncthreads();
-        store_stage<M, 8, 256>(a.X1 + ((long long)slot * a.np1 + fi) * M * HP, st, x0, a.ry_log2);
-        __syncthreads();
+        // block-wide store of columns x0 .. x0+7 into the band layout:
+        // element e = (band b, column c, row r), e = b 8RY + c RY + r
+        float2* plane = a.X1 + ((long long)slot * a.np1 + fi) * M * HP;
+        for (int e = threadIdx.x; e < 4 * M; e += 256) {
+            const int b = e >> (a.ry_log2 + 3), c = (e >> a.ry_log2) & 7, r = e & (RY - 1);
+            stu(plane, ((b * M + x0) << a.ry_log2) + (e & (8 * RY - 1)),
+                lds[c >> 1][(c & 1) * H + b * RY + r]);
+        }
+        if (threadIdx.x < 8)  // the Nyquist row k = M/2
+            stu(plane, x1_index<M>(M / 2, x0 + threadIdx.x, a.ry_log2),
+                lds[threadIdx.x >> 1][(threadIdx.x & 1) * H + M / 2]);
     }
 }
 
