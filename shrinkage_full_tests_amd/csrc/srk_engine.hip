@@ -178,9 +178,7 @@ __device__ __forceinline__ float2 grad_in(const Col3& f, unsigned yy)
 
 // ---------------------------------------------------------------------------
 // P1: real FFT along y of the convolved fields; one wavefront per column pair,
-// 8 columns per block.  Each wavefront unpacks its pair's half spectra into
-// its own LDS line, then the block stores the 8 columns of every band as one
-// contiguous run of the X1 band layout.
+// 8 columns per block.
 // ---------------------------------------------------------------------------
 template <int M, class PL, int NF>
 __global__ __launch_bounds__(256) void k_p1_cols(EngineArgs a)
@@ -270,7 +268,8 @@ __global__ __launch_bounds__(256) void k_p1_cols(EngineArgs a)
 
 // ---------------------------------------------------------------------------
 // P1A: adjoint source a_d = v0 * d_d f0 (+ v1 * d_d f1) on the valid region,
-// transformed along y as a_x + i a_y; one wavefront per column.  Inside the
+// transformed along y as a_x + i a_y; one wavefront per column, 4 per block,
+// band stores staged in the wavefronts' own LDS lines as in P1.  Inside the
 // valid region every stencil neighbour lies inside the image (e >= 1), so the
 // loads need no bounds checks: rows outside it load at a clamped row and are
 // masked.
@@ -288,41 +287,77 @@ __global__ __launch_bounds__(256) void k_p1_adj_src(EngineArgs a, int plane0)
     const int N = a.N, e = a.e, W = a.W;
     const int x = blockIdx.x * 4 + w;
     const int gx = t.p0x + x;
-    float2* planeX = a.X1 + ((long long)slot * a.np1 + plane0) * M * HP;
-    float2* planeY = planeX + (long long)M * HP;
+    constexpr int H = M / 2 + 1;
+    constexpr int KV = (H + kWave - 1) / kWave;
+    float2* line = lds[w];
+    // the wave's line ends up holding P = A_x (k <= M/2) at [k] and Q = A_y at [H + k]
     if (gx < e || gx >= N - e) {
-        zero_store<M>(planeX, x, a.ry_log2, lane);
-        zero_store<M>(planeY, x, a.ry_log2, lane);
-        return;
-    }
-    float2 v[V];
+        for (int k = lane; k < 2 * H; k += kWave) line[k] = make_float2(0.f, 0.f);
+    } else {
+        float2 v[V];
 #pragma unroll
-    for (int i = 0; i < V; ++i) v[i] = make_float2(0.f, 0.f);
-    for (int fi = 0; fi < nf; ++fi) {
-        // column pointers are wavefront-uniform; row offsets are unsigned 32-bit
-        const float* fl = (fi == 0 ? a.f0 : a.f1) + (long long)t.b * a.img_stride +
-                          (long long)(gx - 1) * N;
-        const float* fc = fl + N;
-        const float* fr = fc + N;
-        const float* vc = (fi == 0 ? a.v0 : a.v1) + (long long)t.b * a.w_stride +
-                          (long long)(gx - e) * W - e;
+        for (int i = 0; i < V; ++i) v[i] = make_float2(0.f, 0.f);
+        for (int fi = 0; fi < nf; ++fi) {
+            // column pointers are wavefront-uniform; row offsets are unsigned 32-bit
+            const float* fl = (fi == 0 ? a.f0 : a.f1) + (long long)t.b * a.img_stride +
+                              (long long)(gx - 1) * N;
+            const float* fc = fl + N;
+            const float* fr = fc + N;
+            const float* vc = (fi == 0 ? a.v0 : a.v1) + (long long)t.b * a.w_stride +
+                              (long long)(gx - e) * W - e;
 #pragma unroll
-        for (int i = 0; i < V; ++i) {
-            const int gy = t.p0y + lane + kWave * i;
-            const bool ok = gy >= e && gy < N - e;
-            const unsigned yy = ok ? gy : e;
-            const float c = ldu(vc, yy);
-            const float dxf = 0.5f * (ldu(fr, yy) - ldu(fl, yy));
-            const float dyf = 0.5f * (ldu(fc, yy + 1u) - ldu(fc, yy - 1u));
-            v[i].x += ok ? c * dxf : 0.f;
-            v[i].y += ok ? c * dyf : 0.f;
+            for (int i = 0; i < V; ++i) {
+                const int gy = t.p0y + lane + kWave * i;
+                const bool ok = gy >= e && gy < N - e;
+                const unsigned yy = ok ? gy : e;
+                const float c = ldu(vc, yy);
+                const float dxf = 0.5f * (ldu(fr, yy) - ldu(fl, yy));
+                const float dyf = 0.5f * (ldu(fc, yy + 1u) - ldu(fc, yy - 1u));
+                v[i].x += ok ? c * dxf : 0.f;
+                v[i].y += ok ? c * dyf : 0.f;
+            }
+        }
+        PL FP;
+        FP.load(a.tw, lane);
+        fft_from_regs<M, -1>(FP, line, v, lane);
+        float2 P[KV], Q[KV];
+#pragma unroll
+        for (int i = 0; i < KV; ++i) {
+            const int k = lane + kWave * i;
+            if (k < H) {
+                const float2 z = line[lslot(k)];
+                const float2 zm = line[lslot((M - k) & (M - 1))];
+                P[i] = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
+                Q[i] = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < KV; ++i) {
+            const int k = lane + kWave * i;
+            if (k < H) {
+                line[k] = P[i];
+                line[H + k] = Q[i];
+            }
         }
     }
-    float2* line = lds[w];
-    PL FP;
-    FP.load(a.tw, lane);
-    fft_from_regs<M, -1>(FP, line, v, lane);
-    unpack_store<M>(line, planeX, x, planeY, x, a.ry_log2, lane);
+    __syncthreads();
+    // block-wide store of columns x0 .. x0+3 of both planes into the band layout:
+    // element e = (band b, column c, row r), e = b 4RY + c RY + r
+    const int x0 = blockIdx.x * 4;
+    const int RY = 1 << a.ry_log2;
+    float2* planeX = a.X1 + ((long long)slot * a.np1 + plane0) * M * HP;
+#pragma unroll
+    for (int pl = 0; pl < 2; ++pl) {
+        float2* plane = planeX + (long long)pl * M * HP;
+        for (int e2 = threadIdx.x; e2 < 2 * M; e2 += 256) {
+            const int b = e2 >> (a.ry_log2 + 2), c = (e2 >> a.ry_log2) & 3, r = e2 & (RY - 1);
+            stu(plane, ((b * M + x0) << a.ry_log2) + (e2 & (4 * RY - 1)),
+                lds[c][pl * H + b * RY + r]);
+        }
+        if (threadIdx.x < 4)  // the Nyquist row k = M/2
+            stu(plane, x1_index<M>(M / 2, x0 + threadIdx.x, a.ry_log2),
+                lds[threadIdx.x][pl * H + M / 2]);
+    }
 }
 
 // ---------------------------------------------------------------------------
