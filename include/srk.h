@@ -124,7 +124,7 @@ int srkWorkspaceSetChunk(srk_workspace_t ws, int tiles_per_chunk); /* 0 = automa
 enum {
     SRK_TUNE_CHUNK_TILES = 0,  /* tiles per launch chunk (exchange buffers bound it)   */
     SRK_TUNE_P2_ROWS = 1,      /* k_y bands per row-pass block: 16, 8 or 4 (M <= 512) */
-    SRK_TUNE_CONCURRENCY = 2   /* 1 (default): adjoint front passes on two streams; 0: one */
+    SRK_TUNE_CONCURRENCY = 2   /* 1 (default): chunks alternate between two streams; 0: one */
 };
 int srkWorkspaceSetTuning(srk_workspace_t ws, int key, int value);
 /* SRK_PREC_FP32 (default) or SRK_PREC_MIXED_FP16 for every later call on ws */

@@ -90,7 +90,7 @@ class Workspace:
 
     def set_tuning(self, chunk_tiles=None, p2_rows=None, concurrency=None) -> None:
         """Autotuner knobs: tiles per launch chunk, k_y bands per row-pass block,
-        two-stream adjoint front passes (1, the default) or one stream (0)."""
+        chunks alternating between two streams (1, the default) or one (0)."""
         if chunk_tiles is not None:
             call("srkWorkspaceSetTuning", self.handle, 0, int(chunk_tiles))
         if p2_rows is not None:

@@ -257,8 +257,8 @@ struct srk_workspace_s {
     int precision = SRK_PREC_FP32;
     bool timing = false;
     srk::LaunchTimer timer;
-    bool concurrent = true;  // adjoint front passes on two streams
-    srk::Fork fork{nullptr, nullptr, nullptr};
+    bool concurrent = true;  // chunks alternate between two lanes (streams + buffer halves)
+    srk::Fork fork{nullptr, nullptr, nullptr, nullptr, nullptr};
     srk::Tile* d_ftiles = nullptr;
     srk::Tile* d_atiles = nullptr;
     int nft = 0, nat = 0, S2f = 0, S2a = 0;
@@ -292,7 +292,9 @@ int chunk_for(const srk_workspace_s* w, int np1, int np2, int S2, int ntiles)
     const int HP = srk::spec_pitch(w->M);
     const size_t per1 = (size_t)np1 * w->M * HP * sizeof(float2);
     const size_t per2 = (size_t)np2 * S2 * HP * sizeof(float2);
-    const size_t cap = std::min(w->x1_bytes / per1, w->x2_bytes / per2);
+    // with two lanes each gets half of the exchange buffers
+    const size_t lanes = w->concurrent ? 2 : 1;
+    const size_t cap = std::min(w->x1_bytes / lanes / per1, w->x2_bytes / lanes / per2);
     size_t c = w->chunk_override > 0 ? std::min<size_t>(w->chunk_override, cap) : cap;
     c = std::max<size_t>(1, std::min<size_t>(c, (size_t)ntiles));
     return (int)c;
@@ -363,6 +365,16 @@ srk::LaunchTimer* timer_for(srk_workspace_s* w, hipStream_t s)
     return &w->timer;
 }
 
+// The second lane (stream, event pair and the upper halves of the exchange
+// buffers), or none.
+const srk::Fork* lanes_for(srk_workspace_s* w)
+{
+    if (!w->concurrent) return nullptr;
+    w->fork.X1b = w->X1 + w->x1_bytes / 2 / sizeof(float2);
+    w->fork.X2b = w->X2 + w->x2_bytes / 2 / sizeof(float2);
+    return &w->fork;
+}
+
 int run_fwd(srk_workspace_s* w, srk::Op op, srk::EngineArgs a, int B, hipStream_t s)
 {
     const int nf = op == srk::Op::Tangent ? 2 : 1;
@@ -373,8 +385,8 @@ int run_fwd(srk_workspace_s* w, srk::Op op, srk::EngineArgs a, int B, hipStream_
     a.halo2 = 0;
     const int ntiles = B * w->nft;
     const int chunk = chunk_for(w, a.np1, a.np2, a.S2, ntiles);
-    return hip_status(
-        srk::engine_run(op, w->M, w->precision, a, ntiles, chunk, s, timer_for(w, s)));
+    return hip_status(srk::engine_run(op, w->M, w->precision, a, ntiles, chunk, s,
+                                      timer_for(w, s), lanes_for(w)));
 }
 
 int run_adj(srk_workspace_s* w, srk::Op op, srk::EngineArgs a, int B, hipStream_t s)
@@ -391,7 +403,7 @@ int run_adj(srk_workspace_s* w, srk::Op op, srk::EngineArgs a, int B, hipStream_
     const int ntiles = B * w->nat;
     const int chunk = chunk_for(w, a.np1, a.np2, a.S2, ntiles);
     return hip_status(srk::engine_run(op, w->M, w->precision, a, ntiles, chunk, s,
-                                      timer_for(w, s), w->concurrent ? &w->fork : nullptr));
+                                      timer_for(w, s), lanes_for(w)));
 }
 
 }  // namespace

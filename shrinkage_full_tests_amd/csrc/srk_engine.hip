@@ -910,7 +910,7 @@ hipError_t prepare_kernels()
 }
 
 template <int M, class PL, int RY>
-int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s, LaunchTimer* tm,
+int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s0, LaunchTimer* tm,
              const Fork* fk)
 {
     static_assert(RY == 16 || RY == 8 || RY == 4, "band heights");
@@ -919,10 +919,23 @@ int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s, 
     if (st != hipSuccess) return (int)st;
     const size_t lds2 = p2_lds<M, RY>();
     const int cols_out = a0.S2 - 2 * a0.halo2;  // max output columns per tile
-    for (int t0 = 0; t0 < ntiles; t0 += chunk) {
+    bool aux_used = false;
+    for (int t0 = 0, ci = 0; t0 < ntiles; t0 += chunk, ++ci) {
         const int c = std::min(chunk, ntiles - t0);
         EngineArgs a = a0;
         a.tiles = a0.tiles + t0;
+        // lane of this chunk: odd chunks run on the second lane when there is one
+        const bool lane1 = fk && (ci & 1);
+        const hipStream_t s = lane1 ? fk->aux : s0;
+        if (lane1) {
+            a.X1 = fk->X1b;
+            a.X2 = fk->X2b;
+            if (!aux_used && st == hipSuccess) {
+                st = hipEventRecord(fk->fork, s0);
+                if (st == hipSuccess) st = hipStreamWaitEvent(fk->aux, fk->fork, 0);
+                aux_used = true;
+            }
+        }
         a.ry_log2 = RY == 16 ? 4 : RY == 8 ? 3 : 2;
         const dim3 g1(M / 8, c), g1a(M / 4, c);
         const dim3 g3f((cols_out + 3) / 4, c), g3a((cols_out + 2 * P3A_WAVES - 1) / (2 * P3A_WAVES), c);
@@ -977,24 +990,6 @@ int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s, 
                     k_p3_adj<M, PL, 2><<<g3a, b3a, 0, s>>>(a, with_u, planeL);
             });
         };
-        // adjoint operators: convolved fields (P1 + P2 conv) and adjoint sources
-        // (P1A + P2 corr) are independent until P3 and may run concurrently
-        auto adjoint_front = [&](int nf, int plane_src, int plane_corr) {
-            hipStream_t sa = s;
-            if (fk && st == hipSuccess) {
-                st = hipEventRecord(fk->fork, s);
-                if (st == hipSuccess) st = hipStreamWaitEvent(fk->aux, fk->fork, 0);
-                sa = fk->aux;
-            }
-            p1(nf, sa);
-            p2c(nf, sa);
-            p1a(nf, plane_src, s);
-            p2r(plane_src, plane_corr, s);
-            if (fk && st == hipSuccess) {
-                st = hipEventRecord(fk->join, fk->aux);
-                if (st == hipSuccess) st = hipStreamWaitEvent(s, fk->join, 0);
-            }
-        };
         switch (op) {
         case Op::Forward:
             p1(1, s);
@@ -1007,7 +1002,10 @@ int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s, 
             p3f(2);
             break;
         case Op::Adjoint:
-            adjoint_front(1, 1, 2);
+            p1(1, s);
+            p1a(1, 1, s);
+            p2c(1, s);
+            p2r(1, 2, s);
             p3a(1, 1, 2);
             break;
         case Op::AdjointG:
@@ -1016,7 +1014,10 @@ int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s, 
             p3a(1, 0, 0);
             break;
         case Op::TangentAdjoint:
-            adjoint_front(2, 2, 4);
+            p1(2, s);
+            p1a(2, 2, s);
+            p2c(2, s);
+            p2r(2, 4, s);
             p3a(2, 1, 4);
             break;
         }
@@ -1024,7 +1025,11 @@ int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s, 
         st = hipGetLastError();
         if (st != hipSuccess) return (int)st;
     }
-    return 0;
+    if (aux_used) {
+        st = hipEventRecord(fk->join, fk->aux);
+        if (st == hipSuccess) st = hipStreamWaitEvent(s0, fk->join, 0);
+    }
+    return (int)st;
 }
 
 // P2 band height: the tuned value if it is one of the compiled variants,

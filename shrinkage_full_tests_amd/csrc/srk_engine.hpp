@@ -113,20 +113,23 @@ struct LaunchTimer {
     ~LaunchTimer();
 };
 
-// Second stream for the independent halves of the adjoint passes: the
-// convolved field (P1, P2 conv) runs on `aux` while the adjoint sources
-// (P1A, P2 corr) run on the caller's stream; `fork` / `join` order them
-// (event record + stream wait, so the pattern can be captured in a graph).
+// Second lane for chunk pipelining: chunks alternate between the caller's
+// stream (exchange buffers args.X1 / args.X2) and `aux` (buffers X1b / X2b),
+// so the passes of one chunk overlap those of the next.  `fork` / `join`
+// order aux after the work already on the caller's stream and the caller's
+// stream after aux (event record + stream wait, so it can be captured in a
+// graph).
 struct Fork {
     hipStream_t aux;
     hipEvent_t fork, join;
+    float2* X1b;
+    float2* X2b;
 };
 
 // Launch the passes of `op` for tiles [0, ntiles) of the tile table
 // args.tiles, chunk tiles at a time.  prec 0: FP32 transforms; 1: FP16 MFMA
-// DFT stages with FP32 accumulation (M <= 1024).  With `fork`, the adjoint
-// operators split their first two passes over two streams.  Returns a
-// hipError_t.
+// DFT stages with FP32 accumulation (M <= 1024).  With `fork`, chunks
+// alternate between two lanes.  Returns a hipError_t.
 int engine_run(Op op, int M, int prec, const EngineArgs& args, int ntiles, int chunk,
                hipStream_t stream, LaunchTimer* timer = nullptr, const Fork* fork = nullptr);
 

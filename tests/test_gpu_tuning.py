@@ -58,9 +58,10 @@ def test_band_heights_and_chunks(orc, N, M, kw, prec, rows):
 
 
 def test_two_stream_adjoint_equals_one_stream():
-    """The adjoint front passes on two streams give bit-identical results to
-    one stream (same kernels, disjoint exchange planes), eagerly and when the
-    forked call is captured into a graph and replayed."""
+    """Chunks alternating between two streams (each with its own half of the
+    exchange buffers) give bit-identical results to one stream, for the
+    forward and the adjoint, eagerly and when the forked call is captured
+    into a graph and replayed."""
     import torch
     N, M, kw, B = 600, 256, 31, 3
     e = (kw - 1) // 2
@@ -71,12 +72,14 @@ def test_two_stream_adjoint_equals_one_stream():
     v = dev(np.random.default_rng(3).standard_normal((B, W, W)))
     ws = srk.Workspace(N, W, e, M, B_max=B)
     ws.set_tuning(chunk_tiles=4)
-    out = {}
+    out, hs = {}, {}
     for conc in (0, 1):
         ws.set_tuning(concurrency=conc)
+        hs[conc] = srk.forward_batched(g, f, gx, gy, e, 1.0, ws)
         lf, lg = srk.adjoint_batched(v, g, f, gx, gy, e, 1.0, ws)
         torch.cuda.synchronize()
         out[conc] = (lf.clone(), lg.clone())
+    assert torch.equal(hs[0], hs[1])
     assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
     lf = torch.empty_like(out[1][0])
     lg = torch.empty_like(out[1][1])
