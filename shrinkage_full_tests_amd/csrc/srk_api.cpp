@@ -523,7 +523,7 @@ int srkWorkspaceCreate(int N, int W, int erosion, int M, int B_max, size_t scrat
     const int HP = srk::spec_pitch(M);
     const size_t x1_per = (size_t)4 * M * HP * sizeof(float2);
     const size_t x2_per = (size_t)std::max(4 * S2f, 5 * S2a) * HP * sizeof(float2);
-    const size_t budget = (scratch_mb ? scratch_mb : 1024) << 20;
+    const size_t budget = (scratch_mb ? scratch_mb : 4096) << 20;
     size_t cmax = std::max<size_t>(1, budget / (x1_per + x2_per));
     cmax = std::min<size_t>(cmax, (size_t)B_max * std::max(w->nft, w->nat));
     w->x1_bytes = cmax * x1_per;

@@ -27,14 +27,6 @@ __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 // wavefront index in the block, as a scalar (per-wave pointers stay in SGPRs)
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x / kWave); }
 
-// P1 output staging.  A P1 block transforms CB (8 or 4) consecutive tile
-// columns and stages their half spectra in LDS as st[k * (CB + 1) + c]
-// (k <= M/2, c < CB; the odd pitch keeps the lane-per-k writes on distinct
-// banks), then stores them block-wide into the band layout of X1, where the
-// CB columns of one band are CB RY contiguous elements.
-template <int M, int CB>
-constexpr int stage_elems() { return (M / 2 + 1) * (CB + 1); }
-
 // base[idx] for a wavefront-uniform base and a per-lane index whose byte
 // offset fits in 32 bits: the address becomes base + zext(offset), which the
 // compiler emits as a scalar-base load with a 32-bit vector offset.
@@ -48,66 +40,6 @@ template <class T>
 __device__ __forceinline__ void stu(T* base, unsigned idx, T v)
 {
     *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + idx * (unsigned)sizeof(T)) = v;
-}
-
-// Unpack the transform Z of two real sequences (p + i q) into the half spectra
-// P(k) = (Z(k) + conj Z(M-k))/2 and Q(k) = (Z(k) - conj Z(M-k))/(2i), k <= M/2,
-// staged at column cp of sP and column cq of sQ.
-template <int M, int CB>
-__device__ __forceinline__ void unpack_stage(const float2* line, float2* sP, int cp, float2* sQ,
-                                             int cq, int lane)
-{
-    for (int k = lane; k <= M / 2; k += kWave) {
-        const float2 z = line[lslot(k)];
-        const float2 zm = line[lslot((M - k) & (M - 1))];
-        sP[k * (CB + 1) + cp] = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
-        sQ[k * (CB + 1) + cq] = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
-    }
-}
-
-// Unpack as above, stored straight into X1 band layout (see x1_index) at
-// column xp of planeP and column xq of planeQ.
-template <int M>
-__device__ __forceinline__ void unpack_store(const float2* line, float2* planeP, int xp,
-                                             float2* planeQ, int xq, int rs, int lane)
-{
-    for (int k = lane; k <= M / 2; k += kWave) {
-        const float2 z = line[lslot(k)];
-        const float2 zm = line[lslot((M - k) & (M - 1))];
-        stu(planeP, x1_index<M>(k, xp, rs), make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y)));
-        stu(planeQ, x1_index<M>(k, xq, rs), make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x)));
-    }
-}
-
-template <int M>
-__device__ __forceinline__ void zero_store(float2* plane, int x, int rs, int lane)
-{
-    for (int k = lane; k <= M / 2; k += kWave) stu(plane, x1_index<M>(k, x, rs), make_float2(0.f, 0.f));
-}
-
-template <int M, int CB>
-__device__ __forceinline__ void zero_stage(float2* st, int c, int lane)
-{
-    for (int k = lane; k <= M / 2; k += kWave) st[k * (CB + 1) + c] = make_float2(0.f, 0.f);
-}
-
-// Block-wide store of the staged columns x0 .. x0+CB-1 into an X1 plane (band
-// layout with bands of 2^rs rows, see x1_index).
-template <int M, int CB, int NT>
-__device__ __forceinline__ void store_stage(float2* __restrict__ plane, const float2* st, int x0,
-                                            int rs)
-{
-    constexpr int LCB = CB == 8 ? 3 : 2;
-    static_assert(CB == 8 || CB == 4, "staged column groups of 8 or 4");
-    const int RY = 1 << rs;
-    // full bands: element e = (band b, column c, row r), e = b CB RY + c RY + r
-    for (int e = threadIdx.x; e < CB * (M / 2); e += NT) {
-        const int b = e >> (rs + LCB), c = (e >> rs) & (CB - 1), r = e & (RY - 1);
-        stu(plane, ((b * M + x0) << rs) + (e & (CB * RY - 1)), st[(b * RY + r) * (CB + 1) + c]);
-    }
-    // the last band holds the Nyquist row k = M/2 only
-    if (threadIdx.x < CB)
-        stu(plane, x1_index<M>(M / 2, x0 + threadIdx.x, rs), st[(M / 2) * (CB + 1) + threadIdx.x]);
 }
 
 // Two Hermitian half spectra P, Q (X2 columns, or the zero column) as loaded
