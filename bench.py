@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--large-n", type=int, default=16384)
     ap.add_argument("--large-m", type=int, default=512)
+    ap.add_argument("--no-ragged", action="store_true",
+                    help="skip the ragged 64-instance fwd+adj+tangent-adjoint point")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="run one plain device copy of this size first (PMC traffic calibration)")
     args = ap.parse_args()
@@ -283,6 +285,33 @@ def main():
                                        "hbm_frac": round(bL / (msM / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
                                        "speedup_vs_fp32": round(msL / msM, 3)}
 
+    ragged_out = None
+    if not args.no_ragged:
+        if not args.no_large:
+            del fL, gL, vL, hL, lfL, lgL, wsL
+        torch.cuda.empty_cache()
+        from shrinkage_full_tests_amd import ragged
+        rb = ragged.RaggedBatch()
+        ragged_out = {"workload": "64 instances, N in {1024, 2048, 4096}, KW in {31, 61, 91, 121} "
+                                  "(seed 4242): fwd + adjoint + tangent-adjoint, one hipGraph",
+                      "pixels": rb.pixels}
+        for prec in ("fp32", "mixed_fp16"):
+            rb.set_precision(prec)
+            gr = rb.capture()
+            torch.cuda.synchronize()
+            start.record(stream)
+            for _ in range(3):
+                gr.replay()
+            stop.record(stream)
+            torch.cuda.synchronize()
+            msR = start.elapsed_time(stop) / 3
+            ragged_out[prec] = {"ms_per_step": round(msR, 3),
+                                "Mpix_per_s": round(rb.pixels / (msR / 1e3) / 1e6, 1)}
+            del gr
+        ragged_out["mixed_speedup_vs_fp32"] = round(
+            ragged_out["fp32"]["ms_per_step"] / ragged_out["mixed_fp16"]["ms_per_step"], 3)
+        del rb
+
     cpu = None if args.no_cpu else cpu_baseline(N, kw, sigma, e)
 
     out = {
@@ -318,6 +347,7 @@ def main():
         "kernels": kernels,
         "mixed_fp16": mixed,
         "secondary": secondary,
+        "ragged": ragged_out,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
