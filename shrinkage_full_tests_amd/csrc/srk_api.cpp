@@ -287,14 +287,25 @@ void ws_free(srk_workspace_s* w)
     delete w;
 }
 
-int chunk_for(const srk_workspace_s* w, int np1, int np2, int S2, int ntiles)
+// Tiles of one lane's exchange buffers for an operator with np1 / np2 planes.
+size_t lane_capacity(const srk_workspace_s* w, int np1, int np2, int S2, int lanes)
 {
     const int HP = srk::spec_pitch(w->M);
     const size_t per1 = (size_t)np1 * w->M * HP * sizeof(float2);
     const size_t per2 = (size_t)np2 * S2 * HP * sizeof(float2);
-    // with two lanes each gets half of the exchange buffers
-    const size_t lanes = w->concurrent ? 2 : 1;
-    const size_t cap = std::min(w->x1_bytes / lanes / per1, w->x2_bytes / lanes / per2);
+    return std::min(w->x1_bytes / lanes / per1, w->x2_bytes / lanes / per2);
+}
+
+// Two lanes (each with half of the exchange buffers) when pipelining is on
+// and half of the buffers holds at least one tile; otherwise one.
+int lane_count(const srk_workspace_s* w, int np1, int np2, int S2)
+{
+    return w->concurrent && lane_capacity(w, np1, np2, S2, 2) >= 1 ? 2 : 1;
+}
+
+int chunk_for(const srk_workspace_s* w, int np1, int np2, int S2, int ntiles)
+{
+    const size_t cap = lane_capacity(w, np1, np2, S2, lane_count(w, np1, np2, S2));
     size_t c = w->chunk_override > 0 ? std::min<size_t>(w->chunk_override, cap) : cap;
     c = std::max<size_t>(1, std::min<size_t>(c, (size_t)ntiles));
     return (int)c;
@@ -367,9 +378,9 @@ srk::LaunchTimer* timer_for(srk_workspace_s* w, hipStream_t s)
 
 // The second lane (stream, event pair and the upper halves of the exchange
 // buffers), or none.
-const srk::Fork* lanes_for(srk_workspace_s* w)
+const srk::Fork* lanes_for(srk_workspace_s* w, const srk::EngineArgs& a)
 {
-    if (!w->concurrent) return nullptr;
+    if (lane_count(w, a.np1, a.np2, a.S2) < 2) return nullptr;
     w->fork.X1b = w->X1 + w->x1_bytes / 2 / sizeof(float2);
     w->fork.X2b = w->X2 + w->x2_bytes / 2 / sizeof(float2);
     return &w->fork;
@@ -386,7 +397,7 @@ int run_fwd(srk_workspace_s* w, srk::Op op, srk::EngineArgs a, int B, hipStream_
     const int ntiles = B * w->nft;
     const int chunk = chunk_for(w, a.np1, a.np2, a.S2, ntiles);
     return hip_status(srk::engine_run(op, w->M, w->precision, a, ntiles, chunk, s,
-                                      timer_for(w, s), lanes_for(w)));
+                                      timer_for(w, s), lanes_for(w, a)));
 }
 
 int run_adj(srk_workspace_s* w, srk::Op op, srk::EngineArgs a, int B, hipStream_t s)
@@ -403,7 +414,7 @@ int run_adj(srk_workspace_s* w, srk::Op op, srk::EngineArgs a, int B, hipStream_
     const int ntiles = B * w->nat;
     const int chunk = chunk_for(w, a.np1, a.np2, a.S2, ntiles);
     return hip_status(srk::engine_run(op, w->M, w->precision, a, ntiles, chunk, s,
-                                      timer_for(w, s), lanes_for(w)));
+                                      timer_for(w, s), lanes_for(w, a)));
 }
 
 }  // namespace

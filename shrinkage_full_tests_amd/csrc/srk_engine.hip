@@ -405,7 +405,7 @@ __device__ __forceinline__ void store_band(float2* __restrict__ plane, const flo
 // P2 conv: x-FFT of each convolved-field band row, products with Gx and Gy in
 // registers, inverse x-FFTs, crop; field fi -> X2 planes 2fi (u_x), 2fi+1 (u_y).
 // ---------------------------------------------------------------------------
-constexpr int P2C_ITEMS = 4;  // items per conv block
+constexpr int P2C_ITEMS = 2;  // items per conv block
 
 template <int M, int RY, int NW, class PL>
 __global__ __launch_bounds__(NW * kWave) void k_p2_conv(EngineArgs a, int nf, int tiles)

@@ -97,3 +97,28 @@ def test_two_stream_adjoint_equals_one_stream():
         graph.replay()
         torch.cuda.synchronize()
         assert torch.equal(lf, out[0][0]) and torch.equal(lg, out[0][1])
+
+
+@pytest.mark.parametrize("scratch_mb", [1, 3, 5])
+def test_small_exchange_budgets_match_the_default(scratch_mb):
+    """Exchange budgets of about one tile: the two-lane split is used only when
+    half of the buffers holds a tile, and every chunking gives the same bits
+    (tiles are independent)."""
+    import torch
+    N, M, kw, B = 600, 256, 31, 2
+    e = (kw - 1) // 2
+    W = N - 2 * e
+    gx, gy = srk.kernel_spectra_srsrk(kw, 5.0, M)
+    f = dev(patterns.grf(N, 41, B=B))
+    g = dev(patterns.grf(N, 42, B=B))
+    v = dev(np.random.default_rng(4).standard_normal((B, W, W)))
+    res = []
+    for mb in (0, scratch_mb):
+        ws = srk.Workspace(N, W, e, M, B_max=B, scratch_mb=mb)
+        h = srk.forward_batched(g, f, gx, gy, e, 1.0, ws)
+        lf, lg = srk.adjoint_batched(v, g, f, gx, gy, e, 1.0, ws)
+        torch.cuda.synchronize()
+        res.append((h, lf, lg, ws.info()))
+    assert res[1][3]["chunk_adj"] <= 2
+    for a, b in zip(res[0][:3], res[1][:3]):
+        assert torch.equal(a, b)
