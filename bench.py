@@ -97,7 +97,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--large-n", type=int, default=16384)
-    ap.add_argument("--large-m", type=int, default=512)
+    ap.add_argument("--large-m", type=int, default=1024)
     ap.add_argument("--no-ragged", action="store_true",
                     help="skip the ragged 64-instance fwd+adj+tangent-adjoint point")
     ap.add_argument("--calib-copy-mb", type=int, default=0,

@@ -62,7 +62,7 @@ def _dot(a, b):
     return float((a.double() * b.double()).sum())
 
 
-@pytest.mark.parametrize("N,kw,M", [(4096, 61, 512), (16384, 121, 1024)])
+@pytest.mark.parametrize("N,kw,M", [(4096, 61, 512), (16384, 121, 1024), (32768, 121, 1024)])
 def test_large_image_windows_and_dot_products(orc, N, kw, M):
     torch = _torch()
     e = (kw - 1) // 2
