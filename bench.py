@@ -93,7 +93,9 @@ def main():
     ap.add_argument("--concurrency", type=int, default=-1,
                     help="adjoint front passes on two streams (1) or one (0); -1 = library default")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--precision", default="both", choices=("fp32", "mixed_fp16", "both"))
+    ap.add_argument("--precision", default="both",
+                    choices=("fp32", "mixed_fp16", "bf16_exchange", "both"),
+                    help="both = FP32 plus the two reduced-precision forms")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--large-n", type=int, default=16384)
@@ -219,21 +221,27 @@ def main():
         del graph
         return ms_, tf, ta, ms_eager, kernels
 
-    mixed = None
-    if args.precision in ("mixed_fp16", "both") and M <= 1024:
-        mms, mtf, mta, _, mk = measure("mixed_fp16")
-        mixed = {"value": round(B * N * N / (mms / 1e3) / 1e6, 1), "unit": "Mpix/s",
-                 "ms_per_step": round(mms, 3), "ms_fwd": round(mtf, 3), "ms_adj": round(mta, 3),
-                 "dtype": "f16 MFMA operands, f32 accumulate",
-                 "tolerance": "tol_rel 1e-2, tol_abs 1e-3 vs FP64 (tests/test_gpu_mixed.py)",
-                 "kernels": mk}
-    if args.precision == "mixed_fp16" and mixed is not None:
-        ms, t_f, t_a, ms_eager, kernels = mms, mtf, mta, None, mk
+    dtypes = {"mixed_fp16": "f16 MFMA operands, f32 accumulate",
+              "bf16_exchange": "f32 transforms, bf16 inter-pass spectra (adjoint u in f32)"}
+    reduced = {}
+    for prec in ("mixed_fp16", "bf16_exchange"):
+        if args.precision not in (prec, "both") or (prec == "mixed_fp16" and M > 1024):
+            continue
+        rms, rtf, rta, _, rk = measure(prec)
+        reduced[prec] = {"value": round(B * N * N / (rms / 1e3) / 1e6, 1), "unit": "Mpix/s",
+                         "ms_per_step": round(rms, 3), "ms_fwd": round(rtf, 3),
+                         "ms_adj": round(rta, 3), "dtype": dtypes[prec],
+                         "tolerance": "tol_rel 1e-2, tol_abs 1e-3 vs FP64 (tests/test_gpu_mixed.py)",
+                         "kernels": rk}
+    if args.precision in reduced:
+        r = reduced[args.precision]
+        ms, t_f, t_a, ms_eager, kernels = (r["ms_per_step"], r["ms_fwd"], r["ms_adj"], None,
+                                           r["kernels"])
     else:
         ms, t_f, t_a, ms_eager, kernels = measure("fp32")
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"]) if kernels else None
-    if mixed is not None:
-        mixed["speedup_vs_fp32"] = round(ms / mixed["ms_per_step"], 3)
+    for r in reduced.values():
+        r["speedup_vs_fp32"] = round(ms / r["ms_per_step"], 3)
     graph = not args.no_graph
 
     value = B * N * N / (ms / 1e3) / 1e6
@@ -278,12 +286,14 @@ def main():
                      "value": round(NL * NL / (msL / 1e3) / 1e6, 1), "unit": "Mpix/s",
                      "ms_per_step": round(msL, 3),
                      "hbm_frac": round(bL / (msL / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
-        if args.precision != "fp32" and ML <= 1024:
-            msM = time_large("mixed_fp16")
-            secondary["mixed_fp16"] = {"value": round(NL * NL / (msM / 1e3) / 1e6, 1),
-                                       "ms_per_step": round(msM, 3),
-                                       "hbm_frac": round(bL / (msM / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-                                       "speedup_vs_fp32": round(msL / msM, 3)}
+        for prec in reduced:
+            if prec == "mixed_fp16" and ML > 1024:
+                continue
+            msM = time_large(prec)
+            secondary[prec] = {"value": round(NL * NL / (msM / 1e3) / 1e6, 1),
+                               "ms_per_step": round(msM, 3),
+                               "hbm_frac": round(bL / (msM / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                               "speedup_vs_fp32": round(msL / msM, 3)}
 
     ragged_out = None
     if not args.no_ragged:
@@ -295,7 +305,7 @@ def main():
         ragged_out = {"workload": "64 instances, N in {1024, 2048, 4096}, KW in {31, 61, 91, 121} "
                                   "(seed 4242): fwd + adjoint + tangent-adjoint, one hipGraph",
                       "pixels": rb.pixels}
-        for prec in ("fp32", "mixed_fp16"):
+        for prec in ("fp32",) + tuple(reduced):
             rb.set_precision(prec)
             gr = rb.capture()
             torch.cuda.synchronize()
@@ -308,8 +318,9 @@ def main():
             ragged_out[prec] = {"ms_per_step": round(msR, 3),
                                 "Mpix_per_s": round(rb.pixels / (msR / 1e3) / 1e6, 1)}
             del gr
-        ragged_out["mixed_speedup_vs_fp32"] = round(
-            ragged_out["fp32"]["ms_per_step"] / ragged_out["mixed_fp16"]["ms_per_step"], 3)
+        for prec in reduced:
+            ragged_out[prec]["speedup_vs_fp32"] = round(
+                ragged_out["fp32"]["ms_per_step"] / ragged_out[prec]["ms_per_step"], 3)
         del rb
 
     cpu = None if args.no_cpu else cpu_baseline(N, kw, sigma, e)
@@ -336,7 +347,7 @@ def main():
                    "N": N, "B": B, "KW": kw, "erosion": e, "W": W, "M": M,
                    "tiles_per_chunk_fwd": info["chunk_fwd"], "tiles_per_chunk_adj": info["chunk_adj"],
                    "hipgraph": graph,
-                   "precision": "fp32" if args.precision != "mixed_fp16" else "mixed_fp16"},
+                   "precision": args.precision if args.precision in reduced else "fp32"},
         "ms_fwd": round(t_f, 3),
         "ms_adj": round(t_a, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -349,7 +360,8 @@ def main():
                      "dominant_kernel": dom},
         "ms_per_step_eager_timed": round(ms_eager, 3) if ms_eager else None,
         "kernels": kernels,
-        "mixed_fp16": mixed,
+        "mixed_fp16": reduced.get("mixed_fp16"),
+        "bf16_exchange": reduced.get("bf16_exchange"),
         "secondary": secondary,
         "ragged": ragged_out,
         "cpu_baseline": cpu,

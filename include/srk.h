@@ -56,8 +56,14 @@ enum {
 /* precision of the tile transforms */
 enum {
     SRK_PREC_FP32 = 0,        /* FP32 radix-4/8/16 transforms on the vector ALUs     */
-    SRK_PREC_MIXED_FP16 = 1   /* DFT stages as FP16 MFMA products, FP32 accumulation,
+    SRK_PREC_MIXED_FP16 = 1,  /* DFT stages as FP16 MFMA products, FP32 accumulation,
                                  FP32 twiddles, per-line power-of-two scaling; M <= 1024 */
+    SRK_PREC_BF16_EXCHANGE = 2 /* FP32 transforms; the spectra exchanged between passes
+                                 through HBM stored as bfloat16 (half the bytes), except
+                                 the adjoints' recomputed convolution u, which stays FP32
+                                 (lambda_f multiplies it by the cotangent).  Meets the
+                                 fp16 tolerance (rel 1e-2, abs 1e-3) on inputs of O(1)
+                                 magnitude, not the fp32 one */
 };
 
 typedef struct srk_workspace_s* srk_workspace_t;
@@ -128,7 +134,8 @@ enum {
     SRK_TUNE_CONCURRENCY = 2   /* 1 (default): chunks alternate between two streams; 0: one */
 };
 int srkWorkspaceSetTuning(srk_workspace_t ws, int key, int value);
-/* SRK_PREC_FP32 (default) or SRK_PREC_MIXED_FP16 for every later call on ws */
+/* SRK_PREC_FP32 (default), SRK_PREC_MIXED_FP16 or SRK_PREC_BF16_EXCHANGE for every
+   later call on ws */
 int srkWorkspaceSetPrecision(srk_workspace_t ws, int precision);
 int srkWorkspaceInfo(srk_workspace_t ws, int* M, int* fwd_tiles_per_image,
                      int* adj_tiles_per_image, int* chunk_fwd, int* chunk_adj);

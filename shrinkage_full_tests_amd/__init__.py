@@ -25,6 +25,8 @@ __all__ = [
 
 SUPPORTED_M = (256, 512, 1024, 2048)
 KERNEL_KINDS = 6  # SRK_KERNEL_KINDS
+# SRK_PREC_FP32, SRK_PREC_MIXED_FP16, SRK_PREC_BF16_EXCHANGE
+PRECISIONS = {"fp32": 0, "mixed_fp16": 1, "bf16_exchange": 2}
 
 
 def _torch():
@@ -99,8 +101,10 @@ class Workspace:
             call("srkWorkspaceSetTuning", self.handle, 2, int(concurrency))
 
     def set_precision(self, precision) -> None:
-        """"fp32" (default) or "mixed_fp16" (FP16 MFMA DFT stages, FP32 accumulation)."""
-        code = {"fp32": 0, "mixed_fp16": 1}.get(precision, precision)
+        """"fp32" (default), "mixed_fp16" (FP16 MFMA DFT stages, FP32 accumulation) or
+        "bf16_exchange" (FP32 transforms, bfloat16 spectra between the passes except
+        the adjoints' recomputed convolution)."""
+        code = PRECISIONS.get(precision, precision)
         call("srkWorkspaceSetPrecision", self.handle, int(code))
         self.precision = precision
 

@@ -664,7 +664,7 @@ const char* srkKernelKindName(int kind)
 int srkWorkspaceSetPrecision(srk_workspace_t w, int precision)
 {
     if (!w) return SRK_ERR_NULL_PTR;
-    if (precision == SRK_PREC_FP32) {
+    if (precision == SRK_PREC_FP32 || precision == SRK_PREC_BF16_EXCHANGE) {
         w->precision = precision;
         return SRK_OK;
     }

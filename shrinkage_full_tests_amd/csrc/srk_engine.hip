@@ -42,16 +42,52 @@ __device__ __forceinline__ void stu(T* base, unsigned idx, T v)
     *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + idx * (unsigned)sizeof(T)) = v;
 }
 
+// Element formats of the exchange planes: FP32 complex (float2, 8 bytes) or
+// a bfloat16 pair (Bf16c, 4 bytes, real part in the low half), rounded to
+// nearest even on store (v_cvt_pk_bf16_f32).  A plane keeps its FP32-sized
+// slot in the exchange buffer, so plane p of slot s starts at the same byte
+// address in either format; a bfloat16 plane uses the first half of its slot.
+struct Bf16c {
+    unsigned v;
+};
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ float2 unpack(float2 s) { return s; }
+__device__ __forceinline__ float2 unpack(Bf16c s)
+{
+    return make_float2(__uint_as_float(s.v << 16), __uint_as_float(s.v & 0xffff0000u));
+}
+template <class S>
+__device__ __forceinline__ S pack(float2 v);
+template <>
+__device__ __forceinline__ float2 pack<float2>(float2 v) { return v; }
+template <>
+__device__ __forceinline__ Bf16c pack<Bf16c>(float2 v)
+{
+    const bf16x2_t b = {(__bf16)v.x, (__bf16)v.y};
+    return {__builtin_bit_cast(unsigned, b)};
+}
+template <class S>
+__device__ __forceinline__ float2 ldx(const S* base, unsigned idx) { return unpack(ldu(base, idx)); }
+template <class S>
+__device__ __forceinline__ void stx(S* base, unsigned idx, float2 v) { stu(base, idx, pack<S>(v)); }
+// plane `index` of an exchange buffer whose planes hold `elems` FP32 elements
+template <class S>
+__device__ __forceinline__ S* plane_at(float2* X, long long index, long long elems)
+{
+    return reinterpret_cast<S*>(X + index * elems);
+}
+
 // Two Hermitian half spectra P, Q (X2 columns, or the zero column) as loaded
 // for the lane's transform inputs k = lane + 64 i (raw, so that a load can be
 // issued one transform ahead), and their packing Z = P + iQ.  The DC and
 // Nyquist bins are taken as real, as a real inverse transform does.
-template <int M>
+template <int M, class S>
 struct PairRaw {
     static constexpr int V = M / kWave;
-    float2 p[V], q[V];
+    S p[V], q[V];
 
-    __device__ __forceinline__ void load(const float2* P, const float2* Q, int lane)
+    __device__ __forceinline__ void load(const S* P, const S* Q, int lane)
     {
 #pragma unroll
         for (int i = 0; i < V; ++i) {
@@ -66,7 +102,7 @@ struct PairRaw {
 #pragma unroll
         for (int i = 0; i < V; ++i) {
             const int k = lane + kWave * i;
-            float2 pp = p[i], qq = q[i];
+            float2 pp = unpack(p[i]), qq = unpack(q[i]);
             if (k == 0 || k == M / 2) {
                 pp.y = 0.f;
                 qq.y = 0.f;
@@ -79,11 +115,11 @@ struct PairRaw {
 
 // Inverse transform of P + iQ; afterwards out[i] = p(y) + i q(y) at tile row
 // y = lane + 64 i.
-template <int M, class PL>
-__device__ __forceinline__ void ifft_pair(const PL& FP, float2* line, const float2* P,
-                                          const float2* Q, float2 (&out)[M / kWave], int lane)
+template <int M, class PL, class S>
+__device__ __forceinline__ void ifft_pair(const PL& FP, float2* line, const S* P, const S* Q,
+                                          float2 (&out)[M / kWave], int lane)
 {
-    PairRaw<M> r;
+    PairRaw<M, S> r;
     r.load(P, Q, lane);
     float2 v[M / kWave];
     r.pack(v, lane);
@@ -112,7 +148,7 @@ __device__ __forceinline__ float2 grad_in(const Col3& f, unsigned yy)
 // P1: real FFT along y of the convolved fields; one wavefront per column pair,
 // 8 columns per block.
 // ---------------------------------------------------------------------------
-template <int M, class PL, int NF>
+template <int M, class PL, int NF, class S>
 __global__ __launch_bounds__(256) void k_p1_cols(EngineArgs a)
 {
     constexpr int nf = NF;
@@ -186,14 +222,14 @@ __global__ __launch_bounds__(256) void k_p1_cols(EngineArgs a)
         __syncthreads();
         // block-wide store of columns x0 .. x0+7 into the band layout:
         // element e = (band b, column c, row r), e = b 8RY + c RY + r
-        float2* plane = a.X1 + ((long long)slot * a.np1 + fi) * M * HP;
+        S* plane = plane_at<S>(a.X1, (long long)slot * a.np1 + fi, (long long)M * HP);
         for (int e = threadIdx.x; e < 4 * M; e += 256) {
             const int b = e >> (a.ry_log2 + 3), c = (e >> a.ry_log2) & 7, r = e & (RY - 1);
-            stu(plane, ((b * M + x0) << a.ry_log2) + (e & (8 * RY - 1)),
+            stx(plane, ((b * M + x0) << a.ry_log2) + (e & (8 * RY - 1)),
                 lds[c >> 1][(c & 1) * H + b * RY + r]);
         }
         if (threadIdx.x < 8)  // the Nyquist row k = M/2
-            stu(plane, x1_index<M>(M / 2, x0 + threadIdx.x, a.ry_log2),
+            stx(plane, x1_index<M>(M / 2, x0 + threadIdx.x, a.ry_log2),
                 lds[threadIdx.x >> 1][(threadIdx.x & 1) * H + M / 2]);
     }
 }
@@ -206,7 +242,7 @@ __global__ __launch_bounds__(256) void k_p1_cols(EngineArgs a)
 // loads need no bounds checks: rows outside it load at a clamped row and are
 // masked.
 // ---------------------------------------------------------------------------
-template <int M, class PL, int NF>
+template <int M, class PL, int NF, class S>
 __global__ __launch_bounds__(256) void k_p1_adj_src(EngineArgs a, int plane0)
 {
     constexpr int nf = NF;
@@ -277,17 +313,16 @@ __global__ __launch_bounds__(256) void k_p1_adj_src(EngineArgs a, int plane0)
     // element e = (band b, column c, row r), e = b 4RY + c RY + r
     const int x0 = blockIdx.x * 4;
     const int RY = 1 << a.ry_log2;
-    float2* planeX = a.X1 + ((long long)slot * a.np1 + plane0) * M * HP;
 #pragma unroll
     for (int pl = 0; pl < 2; ++pl) {
-        float2* plane = planeX + (long long)pl * M * HP;
+        S* plane = plane_at<S>(a.X1, (long long)slot * a.np1 + plane0 + pl, (long long)M * HP);
         for (int e2 = threadIdx.x; e2 < 2 * M; e2 += 256) {
             const int b = e2 >> (a.ry_log2 + 2), c = (e2 >> a.ry_log2) & 3, r = e2 & (RY - 1);
-            stu(plane, ((b * M + x0) << a.ry_log2) + (e2 & (4 * RY - 1)),
+            stx(plane, ((b * M + x0) << a.ry_log2) + (e2 & (4 * RY - 1)),
                 lds[c][pl * H + b * RY + r]);
         }
         if (threadIdx.x < 4)  // the Nyquist row k = M/2
-            stu(plane, x1_index<M>(M / 2, x0 + threadIdx.x, a.ry_log2),
+            stx(plane, x1_index<M>(M / 2, x0 + threadIdx.x, a.ry_log2),
                 lds[threadIdx.x][pl * H + M / 2]);
     }
 }
@@ -357,19 +392,19 @@ constexpr int p2_pitch() { return line_slots(M) + 32 / RY; }
 // Band of one X1 plane (M x RY elements, contiguous) -> registers of the
 // block's threads (fetch_band) -> LDS lines (put_band): line r holds k_y row
 // k0 + r along x.  Rows >= nrows are zero.
-template <int M, int RY, int NW>
+template <int M, int RY, int NW, class S>
 struct BandRegs {
     static constexpr int T = NW * kWave, IT = M * RY / T;
     static_assert((T / RY) % 8 == 0 && (M * RY) % T == 0, "band load shape");
-    float2 v[IT];
+    S v[IT];  // as stored: a bfloat16 prefetch holds half the registers
 
-    __device__ __forceinline__ void fetch(const float2* __restrict__ band, int nrows)
+    __device__ __forceinline__ void fetch(const S* __restrict__ band, int nrows)
     {
         const bool ok = (int)(threadIdx.x % RY) < nrows;
-        const float2* src = band + threadIdx.x;
+        const S* src = band + threadIdx.x;
 #pragma unroll
         for (int i = 0; i < IT; ++i) {
-            v[i] = make_float2(0.f, 0.f);
+            v[i] = pack<S>(make_float2(0.f, 0.f));
             if (ok) v[i] = ldu(src, i * T);
         }
     }
@@ -378,26 +413,26 @@ struct BandRegs {
         constexpr int XS = T / RY, LP = p2_pitch<M, RY>();
         float2* d = L + (threadIdx.x % RY) * LP + lslot(threadIdx.x / RY);
 #pragma unroll
-        for (int i = 0; i < IT; ++i) d[i * slot_off(XS)] = v[i];
+        for (int i = 0; i < IT; ++i) d[i * slot_off(XS)] = unpack(v[i]);
     }
 };
 
 // LDS lines -> X2 plane ([x][k_y], pitch HP) for tile-local columns
 // [lo, lo + cnt); columns outside [0, M) are stored as zero.
-template <int M, int RY, int NW>
-__device__ __forceinline__ void store_band(float2* __restrict__ plane, const float2* L, int k0,
+template <int M, int RY, int NW, class S>
+__device__ __forceinline__ void store_band(S* __restrict__ plane, const float2* L, int k0,
                                            int nrows, int lo, int cnt)
 {
     constexpr int T = NW * kWave, XS = T / RY, HP = spec_pitch(M), LP = p2_pitch<M, RY>();
     const int r = threadIdx.x % RY, xi0 = threadIdx.x / RY;
     if (r >= nrows) return;
     const float2* src = L + r * LP;
-    float2* d = plane + (long long)xi0 * HP + k0 + r;
+    S* d = plane + (long long)xi0 * HP + k0 + r;
     for (int xi = xi0; xi < cnt; xi += XS, d += (long long)XS * HP) {
         const int x = lo + xi;
         float2 v = make_float2(0.f, 0.f);
         if (x >= 0 && x < M) v = src[lslot(x)];
-        *d = v;
+        *d = pack<S>(v);
     }
 }
 
@@ -407,7 +442,7 @@ __device__ __forceinline__ void store_band(float2* __restrict__ plane, const flo
 // ---------------------------------------------------------------------------
 constexpr int P2C_ITEMS = 2;  // items per conv block
 
-template <int M, int RY, int NW, class PL>
+template <int M, int RY, int NW, class PL, class S>
 __global__ __launch_bounds__(NW * kWave) void k_p2_conv(EngineArgs a, int nf, int tiles)
 {
     constexpr int V = M / kWave;
@@ -422,9 +457,10 @@ __global__ __launch_bounds__(NW * kWave) void k_p2_conv(EngineArgs a, int nf, in
     FP.load(a.tw, lane);
     const P2Items<M, RY, P2C_ITEMS> items(tiles, nf);
     auto band_src = [&](int slot, int band, int fi) {
-        return a.X1 + ((long long)slot * a.np1 + fi) * M * HP + (long long)band * RY * M;
+        return plane_at<const S>(a.X1, (long long)slot * a.np1 + fi, (long long)M * HP) +
+               (long long)band * RY * M;
     };
-    BandRegs<M, RY, NW> pf;
+    BandRegs<M, RY, NW, S> pf;
     int slot, band, fi;
     items.get(0, slot, band, fi);
     pf.fetch(band_src(slot, band, fi), items.rows(band));
@@ -465,9 +501,11 @@ __global__ __launch_bounds__(NW * kWave) void k_p2_conv(EngineArgs a, int nf, in
         const Tile t = a.tiles[slot];
         const int lo = t.ox0 - t.p0x - a.halo2;
         const int cnt = t.ox1 - t.ox0 + 2 * a.halo2;
-        float2* dstA = a.X2 + ((long long)slot * a.np2 + 2 * fi) * a.S2 * HP;
-        store_band<M, RY, NW>(dstA, A, k0, nrows, lo, cnt);
-        store_band<M, RY, NW>(dstA + (long long)a.S2 * HP, Bs, k0, nrows, lo, cnt);
+        const long long pe = (long long)a.S2 * HP;
+        store_band<M, RY, NW>(plane_at<S>(a.X2, (long long)slot * a.np2 + 2 * fi, pe), A, k0,
+                              nrows, lo, cnt);
+        store_band<M, RY, NW>(plane_at<S>(a.X2, (long long)slot * a.np2 + 2 * fi + 1, pe), Bs,
+                              k0, nrows, lo, cnt);
     }
 }
 
@@ -475,7 +513,7 @@ __global__ __launch_bounds__(NW * kWave) void k_p2_conv(EngineArgs a, int nf, in
 // P2 corr: x-FFT of the adjoint sources, L = conj(Gx) A_x + conj(Gy) A_y in
 // registers, inverse x-FFT, crop -> X2 plane planeOut.
 // ---------------------------------------------------------------------------
-template <int M, int RY, int NW, class PL>
+template <int M, int RY, int NW, class PL, class S>
 __global__ __launch_bounds__(NW * kWave) void k_p2_corr(EngineArgs a, int planeIn, int planeOut,
                                                         int tiles)
 {
@@ -490,8 +528,9 @@ __global__ __launch_bounds__(NW * kWave) void k_p2_corr(EngineArgs a, int planeI
     PL FP;
     FP.load(a.tw, lane);
     const P2Items<M, RY, 1> items(tiles, 1);
-    auto band_src = [&](int slot, int band) {
-        return a.X1 + ((long long)slot * a.np1 + planeIn) * M * HP + (long long)band * RY * M;
+    auto band_src = [&](int slot, int band, int p) {
+        return plane_at<const S>(a.X1, (long long)slot * a.np1 + planeIn + p, (long long)M * HP) +
+               (long long)band * RY * M;
     };
     // one item per block: a prefetch of two planes would cost the registers
     // of the second block per CU
@@ -501,9 +540,9 @@ __global__ __launch_bounds__(NW * kWave) void k_p2_corr(EngineArgs a, int planeI
         const int k0 = band * RY;
         const int nrows = items.rows(band);
         {
-            BandRegs<M, RY, NW> pa, pb;
-            pa.fetch(band_src(slot, band), nrows);
-            pb.fetch(band_src(slot, band) + (long long)M * HP, nrows);
+            BandRegs<M, RY, NW, S> pa, pb;
+            pa.fetch(band_src(slot, band, 0), nrows);
+            pb.fetch(band_src(slot, band, 1), nrows);
             if (k) __syncthreads();  // the previous item's store has read A
             pa.put(A);
             pb.put(Bs);
@@ -531,8 +570,9 @@ __global__ __launch_bounds__(NW * kWave) void k_p2_corr(EngineArgs a, int planeI
         const Tile t = a.tiles[slot];
         const int lo = t.ox0 - t.p0x - a.halo2;
         const int cnt = t.ox1 - t.ox0 + 2 * a.halo2;
-        store_band<M, RY, NW>(a.X2 + ((long long)slot * a.np2 + planeOut) * a.S2 * HP, A, k0,
-                              nrows, lo, cnt);
+        store_band<M, RY, NW>(plane_at<S>(a.X2, (long long)slot * a.np2 + planeOut,
+                                          (long long)a.S2 * HP),
+                              A, k0, nrows, lo, cnt);
     }
 }
 
@@ -540,7 +580,7 @@ __global__ __launch_bounds__(NW * kWave) void k_p2_corr(EngineArgs a, int planeI
 // P3 forward / tangent: inverse y-FFT of (u_x + i u_y) per output column,
 // gradient stencil and combine, valid-region store.
 // ---------------------------------------------------------------------------
-template <int M, class PL, int NF>
+template <int M, class PL, int NF, class S>
 __global__ __launch_bounds__(256) void k_p3_fwd(EngineArgs a)
 {
     constexpr int V = M / kWave;
@@ -556,8 +596,11 @@ __global__ __launch_bounds__(256) void k_p3_fwd(EngineArgs a)
     const int N = a.N, e = a.e, W = a.W;
     const int gx = t.ox0 + xi;
     float2* line = lds[w];
-    const long long pstride = (long long)a.S2 * HP;
-    const float2* base = a.X2 + (long long)slot * a.np2 * pstride + (long long)xi * HP;
+    // this column of X2 plane p
+    auto col = [&](int p) {
+        return plane_at<const S>(a.X2, (long long)slot * a.np2 + p, (long long)a.S2 * HP) +
+               (long long)xi * HP;
+    };
     // output rows [oy0, oy1) and the column lie inside the valid region, so
     // every gradient stencil is inside the image
     const Col3 F0 = col3(a.f0 + (long long)t.b * a.img_stride, N, gx);
@@ -566,7 +609,7 @@ __global__ __launch_bounds__(256) void k_p3_fwd(EngineArgs a)
 
     // u(g) for this column at tile rows y = lane + 64 j
     float2 u0[V];
-    ifft_pair<M, PL>(FP, line, base, base + pstride, u0, lane);
+    ifft_pair<M, PL>(FP, line, col(0), col(1), u0, lane);
     if constexpr (NF == 1) {
 #pragma unroll
         for (int j = 0; j < V; ++j) {
@@ -590,7 +633,7 @@ __global__ __launch_bounds__(256) void k_p3_fwd(EngineArgs a)
         // tangent: + u(dg) . grad f + u(g) . grad df
         const Col3 F1 = col3(a.f1 + (long long)t.b * a.img_stride, N, gx);
         float2 u1[V];
-        ifft_pair<M, PL>(FP, line, base + 2 * pstride, base + 3 * pstride, u1, lane);
+        ifft_pair<M, PL>(FP, line, col(2), col(3), u1, lane);
 #pragma unroll
         for (int j = 0; j < V; ++j) {
             const int gy = t.p0y + lane + kWave * j;
@@ -618,7 +661,7 @@ __global__ __launch_bounds__(256) void k_p3_fwd(EngineArgs a)
 // ---------------------------------------------------------------------------
 constexpr int P3A_WAVES = 4;
 
-template <int M, class PL, int NF>
+template <int M, class PL, int NF, class SL>
 __global__ __launch_bounds__(P3A_WAVES * kWave)
 __attribute__((amdgpu_waves_per_eu(M <= 512 && NF == 1 && !IsMfmaPlan<PL>::value ? 4 : 1))) void k_p3_adj(EngineArgs a, int with_u, int planeL)
 {
@@ -641,14 +684,17 @@ __attribute__((amdgpu_waves_per_eu(M <= 512 && NF == 1 && !IsMfmaPlan<PL>::value
     const int gx = t.ox0 + j0;
     float2* line = lds[w];
     const long long pstride = (long long)a.S2 * HP;
-    const float2* tbase = a.X2 + (long long)slot * a.np2 * pstride;
     const int ncols = cnt + 2;
     // X2 column spectrum of output column j (tile-local), or the zero column
-    // beyond the stored ones
-    auto col = [&](int plane, int j) -> const float2* {
+    // beyond the stored ones; the u planes are FP32, the lambda_g plane SL
+    auto colS = [&](auto fmt, int plane, int j) {
+        using T = decltype(fmt);
         const int xi2 = j + 1;
-        return xi2 < ncols ? tbase + plane * pstride + (long long)xi2 * HP : a.zcol;
+        return xi2 < ncols ? plane_at<const T>(a.X2, (long long)slot * a.np2 + plane, pstride) +
+                                 (long long)xi2 * HP
+                           : reinterpret_cast<const T*>(a.zcol);
     };
+    auto col = [&](int plane, int j) { return colS(float2{}, plane, j); };
     const long long ib = (long long)t.b * a.img_stride;
 
     if (with_u) {
@@ -735,7 +781,7 @@ __attribute__((amdgpu_waves_per_eu(M <= 512 && NF == 1 && !IsMfmaPlan<PL>::value
     if (j0 < cnt) {
         const bool second = j0 + 1 < cnt;
         float2 l[V];
-        ifft_pair<M, PL>(FP, line, col(planeL, j0), col(planeL, j0 + 1), l, lane);
+        ifft_pair<M, PL>(FP, line, colS(SL{}, planeL, j0), colS(SL{}, planeL, j0 + 1), l, lane);
         float* o1 = a.out1 + ib + (long long)gx * N;
 #pragma unroll
         for (int j = 0; j < V; ++j) {
@@ -825,29 +871,34 @@ constexpr size_t p2_lds()
     return 2ull * RY * p2_pitch<M, RY>() * sizeof(float2);
 }
 
-template <int M, int RY, class PL>
+template <int M, int RY, class PL, class SX>
 hipError_t prepare_kernels()
 {
     static bool done = false;
     static hipError_t st = hipSuccess;
     if (done) return st;
     constexpr int NW = P2Waves<M, RY, PL>::NW;
-    st = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_p2_conv<M, RY, NW, PL>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)p2_lds<M, RY>());
-    if (st == hipSuccess)
-        st = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_p2_corr<M, RY, NW, PL>),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)p2_lds<M, RY>());
+    const void* fns[] = {reinterpret_cast<const void*>(&k_p2_conv<M, RY, NW, PL, float2>),
+                         reinterpret_cast<const void*>(&k_p2_conv<M, RY, NW, PL, SX>),
+                         reinterpret_cast<const void*>(&k_p2_corr<M, RY, NW, PL, SX>)};
+    for (const void* fn : fns)
+        if (st == hipSuccess)
+            st = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)p2_lds<M, RY>());
     done = true;
     return st;
 }
 
-template <int M, class PL, int RY>
+// SX is the element format of the forward and tangent planes and of the
+// correlation planes of the adjoints; the adjoints' convolution planes (the
+// recomputed u, which lambda_f multiplies by the cotangent) stay FP32.
+template <int M, class PL, int RY, class SX>
 int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s0, LaunchTimer* tm,
              const Fork* fk)
 {
     static_assert(RY == 16 || RY == 8 || RY == 4, "band heights");
     constexpr int NW = P2Waves<M, RY, PL>::NW;
-    hipError_t st = prepare_kernels<M, RY, PL>();
+    hipError_t st = prepare_kernels<M, RY, PL, SX>();
     if (st != hipSuccess) return (int)st;
     const size_t lds2 = p2_lds<M, RY>();
     const int cols_out = a0.S2 - 2 * a0.halo2;  // max output columns per tile
@@ -882,61 +933,76 @@ int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s0,
                 tm->recs.push_back({kind, c, e0, e1});
             }
         };
-        auto p1 = [&](int nf, hipStream_t ls) {
+        // x16: the convolution planes of this pass use the format SX (else FP32)
+        auto p1 = [&](int nf, bool x16, hipStream_t ls) {
             run(kP1Cols, ls, [&] {
-                if (nf == 1)
-                    k_p1_cols<M, PL, 1><<<g1, b256, 0, ls>>>(a);
+                if (nf == 1 && x16)
+                    k_p1_cols<M, PL, 1, SX><<<g1, b256, 0, ls>>>(a);
+                else if (nf == 1)
+                    k_p1_cols<M, PL, 1, float2><<<g1, b256, 0, ls>>>(a);
+                else if (x16)
+                    k_p1_cols<M, PL, 2, SX><<<g1, b256, 0, ls>>>(a);
                 else
-                    k_p1_cols<M, PL, 2><<<g1, b256, 0, ls>>>(a);
+                    k_p1_cols<M, PL, 2, float2><<<g1, b256, 0, ls>>>(a);
             });
         };
         auto p1a = [&](int nf, int plane0, hipStream_t ls) {
             run(kP1AdjSrc, ls, [&] {
                 if (nf == 1)
-                    k_p1_adj_src<M, PL, 1><<<g1a, b256, 0, ls>>>(a, plane0);
+                    k_p1_adj_src<M, PL, 1, SX><<<g1a, b256, 0, ls>>>(a, plane0);
                 else
-                    k_p1_adj_src<M, PL, 2><<<g1a, b256, 0, ls>>>(a, plane0);
+                    k_p1_adj_src<M, PL, 2, SX><<<g1a, b256, 0, ls>>>(a, plane0);
             });
         };
-        auto p2c = [&](int nf, hipStream_t ls) {
+        auto p2c = [&](int nf, bool x16, hipStream_t ls) {
             const dim3 g2(P2Items<M, RY, P2C_ITEMS>::blocks(c, nf));
-            run(kP2Conv, ls, [&] { k_p2_conv<M, RY, NW, PL><<<g2, b2, lds2, ls>>>(a, nf, c); });
+            run(kP2Conv, ls, [&] {
+                if (x16)
+                    k_p2_conv<M, RY, NW, PL, SX><<<g2, b2, lds2, ls>>>(a, nf, c);
+                else
+                    k_p2_conv<M, RY, NW, PL, float2><<<g2, b2, lds2, ls>>>(a, nf, c);
+            });
         };
         auto p2r = [&](int in, int out, hipStream_t ls) {
             const dim3 g2(P2Items<M, RY, 1>::blocks(c, 1));
-            run(kP2Corr, ls, [&] { k_p2_corr<M, RY, NW, PL><<<g2, b2, lds2, ls>>>(a, in, out, c); });
+            run(kP2Corr, ls,
+                [&] { k_p2_corr<M, RY, NW, PL, SX><<<g2, b2, lds2, ls>>>(a, in, out, c); });
         };
         auto p3f = [&](int nf) {
             run(kP3Fwd, s, [&] {
                 if (nf == 1)
-                    k_p3_fwd<M, PL, 1><<<g3f, b256, 0, s>>>(a);
+                    k_p3_fwd<M, PL, 1, SX><<<g3f, b256, 0, s>>>(a);
                 else
-                    k_p3_fwd<M, PL, 2><<<g3f, b256, 0, s>>>(a);
+                    k_p3_fwd<M, PL, 2, SX><<<g3f, b256, 0, s>>>(a);
             });
         };
         auto p3a = [&](int nf, int with_u, int planeL) {
             run(kP3Adj, s, [&] {
                 if (nf == 1)
-                    k_p3_adj<M, PL, 1><<<g3a, b3a, 0, s>>>(a, with_u, planeL);
+                    k_p3_adj<M, PL, 1, SX><<<g3a, b3a, 0, s>>>(a, with_u, planeL);
                 else
-                    k_p3_adj<M, PL, 2><<<g3a, b3a, 0, s>>>(a, with_u, planeL);
+                    k_p3_adj<M, PL, 2, SX><<<g3a, b3a, 0, s>>>(a, with_u, planeL);
             });
         };
         switch (op) {
         case Op::Forward:
-            p1(1, s);
-            p2c(1, s);
-            p3f(1);
+            // u requested as an output (for a later adjoint) is computed in FP32
+            p1(1, a.ux_out == nullptr, s);
+            p2c(1, a.ux_out == nullptr, s);
+            if (a.ux_out)
+                run(kP3Fwd, s, [&] { k_p3_fwd<M, PL, 1, float2><<<g3f, b256, 0, s>>>(a); });
+            else
+                p3f(1);
             break;
         case Op::Tangent:
-            p1(2, s);
-            p2c(2, s);
+            p1(2, true, s);
+            p2c(2, true, s);
             p3f(2);
             break;
         case Op::Adjoint:
-            p1(1, s);
+            p1(1, false, s);
             p1a(1, 1, s);
-            p2c(1, s);
+            p2c(1, false, s);
             p2r(1, 2, s);
             p3a(1, 1, 2);
             break;
@@ -946,9 +1012,9 @@ int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s0,
             p3a(1, 0, 0);
             break;
         case Op::TangentAdjoint:
-            p1(2, s);
+            p1(2, false, s);
             p1a(2, 2, s);
-            p2c(2, s);
+            p2c(2, false, s);
             p2r(2, 4, s);
             p3a(2, 1, 4);
             break;
@@ -966,15 +1032,15 @@ int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s0,
 
 // P2 band height: the tuned value if it is one of the compiled variants,
 // otherwise the default for M.
-template <int M, class PL>
+template <int M, class PL, class SX = float2>
 int run_rows(Op op, const EngineArgs& a, int ntiles, int chunk, int rows, hipStream_t s,
              LaunchTimer* tm, const Fork* fk)
 {
     if constexpr (M <= 512) {
-        if (rows == 16) return run_impl<M, PL, 16>(op, a, ntiles, chunk, s, tm, fk);
-        if (rows == 4) return run_impl<M, PL, 4>(op, a, ntiles, chunk, s, tm, fk);
+        if (rows == 16) return run_impl<M, PL, 16, SX>(op, a, ntiles, chunk, s, tm, fk);
+        if (rows == 4) return run_impl<M, PL, 4, SX>(op, a, ntiles, chunk, s, tm, fk);
     }
-    return run_impl<M, PL, P2Cfg<M>::RY>(op, a, ntiles, chunk, s, tm, fk);
+    return run_impl<M, PL, P2Cfg<M>::RY, SX>(op, a, ntiles, chunk, s, tm, fk);
 }
 
 }  // namespace
@@ -999,6 +1065,16 @@ int engine_run(Op op, int M, int prec, const EngineArgs& a, int ntiles, int chun
         case 256: return run_rows<256, FftPlanMfma<256>>(op, a, ntiles, chunk, rows, s, tm, fk);
         case 512: return run_rows<512, FftPlanMfma<512>>(op, a, ntiles, chunk, rows, s, tm, fk);
         case 1024: return run_rows<1024, FftPlanMfma<1024>>(op, a, ntiles, chunk, rows, s, tm, fk);
+        default: return (int)hipErrorInvalidValue;
+        }
+    }
+    if (prec == 2) {
+        using P = Bf16c;
+        switch (M) {
+        case 256: return run_rows<256, FftPlan<256>, P>(op, a, ntiles, chunk, rows, s, tm, fk);
+        case 512: return run_rows<512, FftPlan<512>, P>(op, a, ntiles, chunk, rows, s, tm, fk);
+        case 1024: return run_rows<1024, FftPlan<1024>, P>(op, a, ntiles, chunk, rows, s, tm, fk);
+        case 2048: return run_rows<2048, FftPlan<2048>, P>(op, a, ntiles, chunk, rows, s, tm, fk);
         default: return (int)hipErrorInvalidValue;
         }
     }

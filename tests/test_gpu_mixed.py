@@ -1,6 +1,7 @@
-"""Mixed-precision form (FP16 MFMA DFT stages, FP32 accumulation) against the
-FP64 oracle at the fp16 tolerance of README.md:8-10 (tol_rel 1e-2, tol_abs
-1e-3), plus CD parity on line/space patterns (< 0.01 px)."""
+"""Reduced-precision forms against the FP64 oracle at the fp16 tolerance of
+README.md:8-10 (tol_rel 1e-2, tol_abs 1e-3), plus CD parity on line/space
+patterns (< 0.01 px): "mixed_fp16" (FP16 MFMA DFT stages, FP32 accumulation)
+and "bf16_exchange" (FP32 transforms, bfloat16 spectra between the passes)."""
 import numpy as np
 import pytest
 
@@ -11,6 +12,7 @@ from shrinkage_full_tests_amd import patterns
 pytestmark = pytest.mark.gpu
 
 TOL_REL, TOL_ABS = 1e-2, 1e-3
+PRECS = ["mixed_fp16", "bf16_exchange"]
 
 
 def dev(a):
@@ -36,9 +38,10 @@ def kernels(kw, sigma, M):
     return gx, gy, hx.numpy().astype(np.float64), hy.numpy().astype(np.float64)
 
 
+@pytest.mark.parametrize("prec", PRECS)
 @pytest.mark.parametrize("N,kw,M", [(256, 31, 256), (288, 31, 256), (512, 121, 512), (288, 31, 512),
                                     (1024, 61, 1024), (1500, 61, 512)])
-def test_mixed_forward_adjoint(orc, N, kw, M):
+def test_mixed_forward_adjoint(orc, N, kw, M, prec):
     e = (kw - 1) // 2 + (1 if N == 288 else 0)
     W = N - 2 * e
     sigma = ((kw - 1) // 2) / 3.0
@@ -46,7 +49,7 @@ def test_mixed_forward_adjoint(orc, N, kw, M):
     f = patterns.grf(N, 1000 + N)[0]
     g = patterns.grf(N, 2000 + N)[0]
     v = np.random.default_rng(N).standard_normal((W, W)).astype(np.float32)
-    ws = srk.Workspace(N, W, e, M, precision="mixed_fp16")
+    ws = srk.Workspace(N, W, e, M, precision=prec)
     h = srk.forward_batched(dev(g[None]), dev(f[None]), gx, gy, e, 1.0, ws)
     lf, lg = srk.adjoint_batched(dev(v[None]), dev(g[None]), dev(f[None]), gx, gy, e, 1.0, ws)
     fft = N > 600
@@ -56,8 +59,9 @@ def test_mixed_forward_adjoint(orc, N, kw, M):
     assert_close(host(lg)[0], rg, "lambda_g")
 
 
+@pytest.mark.parametrize("prec", PRECS)
 @pytest.mark.parametrize("M", [256, 512])
-def test_mixed_tangent_and_tangent_adjoint(orc, M):
+def test_mixed_tangent_and_tangent_adjoint(orc, M, prec):
     N, e, kw, sigma = 288, 16, 31, 5.0
     W = N - 2 * e
     gx, gy, Kx, Ky = kernels(kw, sigma, M)
@@ -65,7 +69,7 @@ def test_mixed_tangent_and_tangent_adjoint(orc, M):
     rng = np.random.default_rng(200)
     lam = rng.standard_normal((W, W)).astype(np.float32)
     dlam = rng.standard_normal((W, W)).astype(np.float32)
-    ws = srk.Workspace(N, W, e, M, precision="mixed_fp16")
+    ws = srk.Workspace(N, W, e, M, precision=prec)
     dh = dev(np.zeros((W, W)))
     srk.shrinkageTangentOaS(dev(f), dev(g), dev(df), dev(dg), gx, gy, N, W, e, 1.0, ws, dh)
     assert_close(host(dh), orc.tangent(f, g, df, dg, e, Kx, Ky), "dh")
@@ -77,7 +81,8 @@ def test_mixed_tangent_and_tangent_adjoint(orc, M):
     assert_close(host(b), rb, "dlam_g")
 
 
-def test_mixed_cd_box_lines(orc):
+@pytest.mark.parametrize("prec", PRECS)
+def test_mixed_cd_box_lines(orc, prec):
     import torch
     N, kw = 256, 121
     e = 60
@@ -86,7 +91,7 @@ def test_mixed_cd_box_lines(orc):
                      patterns.lns_box(N, duty=0.4, vertical=True)])
     vert = [True, False, True]
     gx, gy, Kx, Ky = kernels(kw, 20.0, N)
-    ws = srk.Workspace(N, W, e, N, B_max=3, precision="mixed_fp16")
+    ws = srk.Workspace(N, W, e, N, B_max=3, precision=prec)
     R = torch.empty((3, W, W), device="cuda")
     dE = dev(imgs)
     srk.srkTrainingForward(dE, dE, gx, gy, N, W, e, 1.0, 3, ws, R)
@@ -107,3 +112,30 @@ def test_mixed_cd_box_lines(orc):
 def test_mixed_precision_rejected_for_large_tiles():
     with pytest.raises(srk.SrkError):
         srk.Workspace(4096, 4096 - 60, 30, 2048, precision="mixed_fp16")
+
+
+def test_bf16_exchange_large_tile_and_stored_u(orc):
+    """M = 2048 tiles (not available to the MFMA form), and the single-image
+    pair srkForwardOaS (u stored, computed in FP32) + adjointSRK (lambda_f from
+    the stored u, lambda_g through the bfloat16 correlation plane)."""
+    N, kw, M = 2300, 61, 2048
+    e = 30
+    W = N - 2 * e
+    gx, gy, Kx, Ky = kernels(kw, 10.0, M)
+    f = patterns.grf(N, 77)[0]
+    g = patterns.grf(N, 78)[0]
+    v = np.random.default_rng(79).standard_normal((W, W)).astype(np.float32)
+    ws = srk.Workspace(N, W, e, M, precision="bf16_exchange")
+    h = srk.forward_batched(dev(g[None]), dev(f[None]), gx, gy, e, 1.0, ws)
+    assert_close(host(h)[0], orc.forward(f, g, e, Kx, Ky, fft=True), "h")
+    rf, rg = orc.adjoint(v, f, g, e, Kx, Ky, fft=True)
+    lf, lg = srk.adjoint_batched(dev(v[None]), dev(g[None]), dev(f[None]), gx, gy, e, 1.0, ws)
+    assert_close(host(lf)[0], rf, "lambda_f")
+    assert_close(host(lg)[0], rg, "lambda_g")
+    ux = dev(np.zeros((W, W))); uy = dev(np.zeros((W, W))); R = dev(np.zeros((W, W)))
+    srk.srkForwardOaS(dev(g), dev(f), gx, gy, N, W, e, 1.0, ws, R, ux, uy)
+    assert_close(host(R), orc.forward(f, g, e, Kx, Ky, fft=True), "h (u stored)")
+    lf2, lg2 = dev(np.zeros((N, N))), dev(np.zeros((N, N)))
+    srk.adjointSRK(dev(v), ux, uy, dev(f), gx, gy, N, W, e, 1.0, ws, lf2, lg2)
+    assert_close(host(lf2), rf, "lambda_f (stored u)")
+    assert_close(host(lg2), rg, "lambda_g (stored u)")

@@ -2,8 +2,9 @@
 {1024, 2048, 4096}, KW in {31, 61, 91, 121} (seed 4242), forward + adjoint +
 tangent-adjoint for all groups captured in one graph.  Checks: graph replay
 equals the eager run bit for bit; FP64-oracle windows for one instance of each
-kernel width; the mixed-precision MFMA form against the FP32 form at the fp16
-tolerance (rel 1e-2, abs 1e-3) on every output."""
+kernel width; the reduced-precision forms (FP16 MFMA stages, bfloat16 exchange)
+against the FP32 form at the fp16 tolerance (rel 1e-2, abs 1e-3) on every
+output."""
 import numpy as np
 import pytest
 
@@ -41,14 +42,15 @@ def test_ragged_64_instances_graph_oracle_and_mixed(orc):
         check_forward_window(orc, s["f"][0], s["g"][0], s["h"][0], e, Kx, Ky, W // 2, 5, 24)
         check_adjoint_window(orc, s["f"][0], s["g"][0], s["v"][0], s["lf"][0], s["lg"][0], e, Kx,
                              Ky, N - 30, N // 3, 24)
-    # mixed-precision form against the FP32 form
-    rb.set_precision("mixed_fp16")
-    rb.run()
-    torch.cuda.synchronize()
-    for s, ref in zip(rb.groups, eager):
-        for k, r in ref.items():
-            a = s[k].double()
-            n = r.double()
-            lim = torch.clamp(1e-2 * torch.maximum(a.abs(), n.abs()), min=1e-3)
-            bad = int(((a - n).abs() > lim).sum())
-            assert bad == 0, (s["N"], s["kw"], k, bad, float((a - n).abs().max()))
+    # reduced-precision forms against the FP32 form
+    for prec in ("mixed_fp16", "bf16_exchange"):
+        rb.set_precision(prec)
+        rb.run()
+        torch.cuda.synchronize()
+        for s, ref in zip(rb.groups, eager):
+            for k, r in ref.items():
+                a = s[k].double()
+                n = r.double()
+                lim = torch.clamp(1e-2 * torch.maximum(a.abs(), n.abs()), min=1e-3)
+                bad = int(((a - n).abs() > lim).sum())
+                assert bad == 0, (prec, s["N"], s["kw"], k, bad, float((a - n).abs().max()))

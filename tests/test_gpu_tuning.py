@@ -1,5 +1,5 @@
 """Every tuning variant of the engine gives the same operator: row-pass band
-heights 16, 8, 4 and small launch chunks, in both precision forms, on whole
+heights 16, 8, 4 and small launch chunks, in every precision form, on whole
 image tiles (N = M) and on overlap-and-save tiles (N > M), against the FP64
 oracle at the tolerance of each form (README.md:8-10)."""
 import numpy as np
@@ -10,7 +10,7 @@ from shrinkage_full_tests_amd import patterns
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"fp32": (1e-3, 1e-5), "mixed_fp16": (1e-2, 1e-3)}
+TOL = {"fp32": (1e-3, 1e-5), "mixed_fp16": (1e-2, 1e-3), "bf16_exchange": (1e-2, 1e-3)}
 
 
 def dev(a):
@@ -33,7 +33,7 @@ def assert_close(a, n, prec, what):
 
 
 @pytest.mark.parametrize("N,M,kw", [(256, 256, 31), (600, 256, 31)])
-@pytest.mark.parametrize("prec", ["fp32", "mixed_fp16"])
+@pytest.mark.parametrize("prec", ["fp32", "mixed_fp16", "bf16_exchange"])
 @pytest.mark.parametrize("rows", [16, 8, 4])
 def test_band_heights_and_chunks(orc, N, M, kw, prec, rows):
     e = (kw - 1) // 2
