@@ -102,6 +102,8 @@ def main():
     ap.add_argument("--large-m", type=int, default=1024)
     ap.add_argument("--no-ragged", action="store_true",
                     help="skip the ragged 64-instance fwd+adj+tangent-adjoint point")
+    ap.add_argument("--no-refcfg", action="store_true",
+                    help="skip the forward at the configuration of the reference's published timing")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="run one plain device copy of this size first (PMC traffic calibration)")
     args = ap.parse_args()
@@ -323,6 +325,41 @@ def main():
                 ragged_out["fp32"]["ms_per_step"] / ragged_out[prec]["ms_per_step"], 3)
         del rb
 
+    refcfg = None
+    if not args.no_refcfg:
+        # The reference's one published timing: srkTrainingForward, B=3000, N=256,
+        # KW=121 (sigma 20, erosion 60, W=136), dx=1, forward only, 55.39 ms per
+        # call on an sm_86 GPU (notebooks/plot_compare_demo.ipynb:20-21).
+        Nr, Br, kwr, er = 256, 3000, 121, 60
+        Wr = Nr - 2 * er
+        gxr, gyr = srk.kernel_spectra_srsrk(kwr, 20.0, Nr)
+        Er = patterns.grf_torch(Br, Nr, seed=5000)
+        Ir = patterns.grf_torch(Br, Nr, seed=6000)
+        Rr = torch.empty((Br, Wr, Wr), device="cuda")
+        wsr = srk.Workspace(Nr, Wr, er, Nr, B_max=Br)
+
+        def fref():
+            srk.srkTrainingForward(Er, Ir, gxr, gyr, Nr, Wr, er, 1.0, Br, wsr, Rr)
+
+        for _ in range(2):
+            fref()
+        torch.cuda.synchronize()
+        start.record(stream)
+        for _ in range(10):
+            fref()
+        stop.record(stream)
+        torch.cuda.synchronize()
+        msr = start.elapsed_time(stop) / 10
+        refcfg = {"workload": "srkTrainingForward B=3000 N=256 KW=121 erosion=60 W=136 dx=1, "
+                              "FP32, forward only (eager calls)",
+                  "ms_per_call": round(msr, 3),
+                  "Mpix_per_s": round(Br * Nr * Nr / (msr / 1e3) / 1e6, 1),
+                  "reference_ms_per_call": 55.39,
+                  "reference_hardware": "NVIDIA sm_86 GPU (published, different hardware)",
+                  "ratio_reference_over_this": round(55.39 / msr, 2)}
+        wsr.close()
+        del Er, Ir, Rr
+
     cpu = None if args.no_cpu else cpu_baseline(N, kw, sigma, e)
 
     out = {
@@ -364,6 +401,7 @@ def main():
         "bf16_exchange": reduced.get("bf16_exchange"),
         "secondary": secondary,
         "ragged": ragged_out,
+        "reference_config_forward": refcfg,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
