@@ -24,10 +24,11 @@ HBM_PEAK_GBS = 8000.0
 PROFILES = os.path.join(ROOT, "profiles")
 
 
-def pmc_traffic(N, B, kw, M):
+def pmc_traffic(N, B, kw, M, precision="fp32"):
     """HBM bytes per fwd+adj step from the committed rocprofv3 --pmc summary of
     this workload (FETCH_SIZE and WRITE_SIZE, copy-calibrated), or None."""
-    path = os.path.join(PROFILES, f"hbm_traffic_fwd_adj_n{N}_b{B}_kw{kw}_m{M}.md")
+    tag = "" if precision == "fp32" else "_" + precision
+    path = os.path.join(PROFILES, f"hbm_traffic_fwd_adj_n{N}_b{B}_kw{kw}_m{M}{tag}.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -242,15 +243,19 @@ def main():
     else:
         ms, t_f, t_a, ms_eager, kernels = measure("fp32")
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"]) if kernels else None
-    for r in reduced.values():
+    for prec, r in reduced.items():
         r["speedup_vs_fp32"] = round(ms / r["ms_per_step"], 3)
+        tr = pmc_traffic(N, B, kw, M, prec)
+        r["traffic"] = tr
+        if tr:
+            r["traffic_TB_per_s"] = round(tr / (r["ms_per_step"] / 1e3) / 1e12, 2)
     graph = not args.no_graph
 
     value = B * N * N / (ms / 1e3) / 1e6
     nbytes = alg_bytes(N, W, B, True)
     achieved = nbytes / (ms / 1e3) / 1e9
     info = ws.info()
-    traffic = pmc_traffic(N, B, kw, M)
+    traffic = pmc_traffic(N, B, kw, M, "fp32")
 
     secondary = None
     if not args.no_large:

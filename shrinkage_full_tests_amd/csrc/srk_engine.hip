@@ -253,6 +253,10 @@ __global__ __launch_bounds__(256) void k_p1_adj_src(EngineArgs a, int plane0)
     const int slot = blockIdx.y;
     const Tile t = a.tiles[slot];
     const int N = a.N, e = a.e, W = a.W;
+    // a block whose four columns all lie off the valid region has only zero
+    // sources; P2 corr does not read those columns
+    const int gb = t.p0x + blockIdx.x * 4;
+    if (gb + 3 < e || gb >= N - e) return;
     const int x = blockIdx.x * 4 + w;
     const int gx = t.p0x + x;
     constexpr int H = M / 2 + 1;
@@ -398,14 +402,19 @@ struct BandRegs {
     static_assert((T / RY) % 8 == 0 && (M * RY) % T == 0, "band load shape");
     S v[IT];  // as stored: a bfloat16 prefetch holds half the registers
 
-    __device__ __forceinline__ void fetch(const S* __restrict__ band, int nrows)
+    // columns outside [xlo, xhi) (tile-local) are taken as zero and not read
+    __device__ __forceinline__ void fetch(const S* __restrict__ band, int nrows, int xlo = 0,
+                                          int xhi = M)
     {
+        constexpr int XS = T / RY;
         const bool ok = (int)(threadIdx.x % RY) < nrows;
+        const int x0 = threadIdx.x / RY;
         const S* src = band + threadIdx.x;
 #pragma unroll
         for (int i = 0; i < IT; ++i) {
+            const int x = x0 + i * XS;
             v[i] = pack<S>(make_float2(0.f, 0.f));
-            if (ok) v[i] = ldu(src, i * T);
+            if (ok && x >= xlo && x < xhi) v[i] = ldu(src, i * T);
         }
     }
     __device__ __forceinline__ void put(float2* L) const
@@ -417,18 +426,19 @@ struct BandRegs {
     }
 };
 
-// LDS lines -> X2 plane ([x][k_y], pitch HP) for tile-local columns
-// [lo, lo + cnt); columns outside [0, M) are stored as zero.
+// LDS lines -> X2 plane ([x][k_y], pitch HP): X2 column xi holds tile-local
+// column lo + xi; columns xi in [xs, xe) are stored, those outside [0, M) as
+// zero.
 template <int M, int RY, int NW, class S>
 __device__ __forceinline__ void store_band(S* __restrict__ plane, const float2* L, int k0,
-                                           int nrows, int lo, int cnt)
+                                           int nrows, int lo, int xs, int xe)
 {
     constexpr int T = NW * kWave, XS = T / RY, HP = spec_pitch(M), LP = p2_pitch<M, RY>();
-    const int r = threadIdx.x % RY, xi0 = threadIdx.x / RY;
+    const int r = threadIdx.x % RY, xi0 = xs + threadIdx.x / RY;
     if (r >= nrows) return;
     const float2* src = L + r * LP;
     S* d = plane + (long long)xi0 * HP + k0 + r;
-    for (int xi = xi0; xi < cnt; xi += XS, d += (long long)XS * HP) {
+    for (int xi = xi0; xi < xe; xi += XS, d += (long long)XS * HP) {
         const int x = lo + xi;
         float2 v = make_float2(0.f, 0.f);
         if (x >= 0 && x < M) v = src[lslot(x)];
@@ -440,22 +450,57 @@ __device__ __forceinline__ void store_band(S* __restrict__ plane, const float2* 
 // P2 conv: x-FFT of each convolved-field band row, products with Gx and Gy in
 // registers, inverse x-FFTs, crop; field fi -> X2 planes 2fi (u_x), 2fi+1 (u_y).
 // ---------------------------------------------------------------------------
-constexpr int P2C_ITEMS = 2;  // items per conv block
+// Large tiles (vector-ALU plan, M >= 1024) keep one LDS line per band row
+// instead of two, so that two blocks fit a CU (74 instead of 148 KB at both
+// M = 1024 and 2048 with their band heights): the corr pass transforms its two
+// source planes one after the other; the conv pass transforms and stores u_x
+// first and then u_y through the same lines, keeping the u_y spectrum in
+// registers meanwhile, with one item per block (no prefetch registers).  At
+// M = 1024 (N = 16384 step) the one-line corr pass took 2.33-2.43 ms against
+// 2.80 for two lines; the conv pass measured the same either way (4.45-4.58 ms).
+enum P2Pass { kP2ConvPass = 0, kP2CorrPass = 1 };
+template <int M, class PL, int PASS>
+constexpr bool p2_one_line()
+{
+    return !IsMfmaPlan<PL>::value && M >= 1024;
+}
+template <int M, class PL>
+constexpr int p2c_items() { return p2_one_line<M, PL, kP2ConvPass>() ? 1 : 2; }
+
+// X2 columns [xs, xe) of a tile's u planes that P3 reads: u enters the
+// operators only multiplied by a field that vanishes off the valid region
+// (grad f in the forward, the cotangent in the adjoints), so only the global
+// columns [e, N - e) are stored.  X2 column xi is global column
+// ox0 - halo2 + xi.
+__device__ __forceinline__ void u_columns(const EngineArgs& a, const Tile& t, int& xs, int& xe)
+{
+    const int cnt = t.ox1 - t.ox0 + 2 * a.halo2;
+    xs = max(0, a.e - t.ox0 + a.halo2);
+    xe = min(cnt, a.N - a.e - t.ox0 + a.halo2);
+}
+
+// two 8-wavefront blocks per CU at M = 1024 need at most 128 registers
+template <int M, class PL, int PASS>
+constexpr int p2_waves_per_eu() { return p2_one_line<M, PL, PASS>() && M == 1024 ? 4 : 1; }
 
 template <int M, int RY, int NW, class PL, class S>
-__global__ __launch_bounds__(NW * kWave) void k_p2_conv(EngineArgs a, int nf, int tiles)
+__global__ __launch_bounds__(NW * kWave)
+__attribute__((amdgpu_waves_per_eu(p2_waves_per_eu<M, PL, kP2ConvPass>())))
+void k_p2_conv(EngineArgs a, int nf, int tiles)
 {
     constexpr int V = M / kWave;
     constexpr int H = M / 2 + 1;
     constexpr int HP = spec_pitch(M);
     constexpr int LP = p2_pitch<M, RY>();
+    constexpr bool ONE = p2_one_line<M, PL, kP2ConvPass>();
+    static_assert(!ONE || NW == RY, "one band row per wavefront");
     extern __shared__ float2 sm[];
     float2* A = sm;
-    float2* Bs = sm + RY * LP;
+    float2* Bs = ONE ? sm : sm + RY * LP;
     const int lane = lane_id(), w = wave_id();
     PL FP;
     FP.load(a.tw, lane);
-    const P2Items<M, RY, P2C_ITEMS> items(tiles, nf);
+    const P2Items<M, RY, p2c_items<M, PL>()> items(tiles, nf);
     auto band_src = [&](int slot, int band, int fi) {
         return plane_at<const S>(a.X1, (long long)slot * a.np1 + fi, (long long)M * HP) +
                (long long)band * RY * M;
@@ -475,6 +520,37 @@ __global__ __launch_bounds__(NW * kWave) void k_p2_conv(EngineArgs a, int nf, in
             int s2, b2, f2;
             items.get(k + 1, s2, b2, f2);
             pf.fetch(band_src(s2, b2, f2), items.rows(b2));
+        }
+        const long long pe = (long long)a.S2 * HP;
+        if constexpr (ONE) {
+            const Tile t = a.tiles[slot];
+            const int lo = t.ox0 - t.p0x - a.halo2;
+            int xs, xe;
+            u_columns(a, t, xs, xe);
+            const bool row = w < nrows;
+            float2* la = A + w * LP;
+            float2 U[V];
+            if (row) {
+                const float2* gxr = a.GT + (long long)(k0 + w) * M + lane;
+                const float2* gyr = a.GT + (long long)(H + k0 + w) * M + lane;
+                float2 X[V];
+                fft_lds_to_regs<M, -1>(FP, la, X, lane);
+#pragma unroll
+                for (int i = 0; i < V; ++i) {
+                    U[i] = cmul(X[i], ldu(gyr, kWave * i));
+                    X[i] = cmul(X[i], ldu(gxr, kWave * i));
+                }
+                fft_from_regs<M, +1>(FP, la, X, lane);
+            }
+            __syncthreads();
+            store_band<M, RY, NW>(plane_at<S>(a.X2, (long long)slot * a.np2 + 2 * fi, pe), A, k0,
+                                  nrows, lo, xs, xe);
+            __syncthreads();  // every row of u_x is stored: the lines take u_y
+            if (row) fft_from_regs<M, +1>(FP, la, U, lane);
+            __syncthreads();
+            store_band<M, RY, NW>(plane_at<S>(a.X2, (long long)slot * a.np2 + 2 * fi + 1, pe), A,
+                                  k0, nrows, lo, xs, xe);
+            continue;
         }
         for (int r = w; r < nrows; r += NW) {
             float2* la = A + r * LP;
@@ -500,12 +576,12 @@ __global__ __launch_bounds__(NW * kWave) void k_p2_conv(EngineArgs a, int nf, in
         __syncthreads();
         const Tile t = a.tiles[slot];
         const int lo = t.ox0 - t.p0x - a.halo2;
-        const int cnt = t.ox1 - t.ox0 + 2 * a.halo2;
-        const long long pe = (long long)a.S2 * HP;
+        int xs, xe;
+        u_columns(a, t, xs, xe);
         store_band<M, RY, NW>(plane_at<S>(a.X2, (long long)slot * a.np2 + 2 * fi, pe), A, k0,
-                              nrows, lo, cnt);
+                              nrows, lo, xs, xe);
         store_band<M, RY, NW>(plane_at<S>(a.X2, (long long)slot * a.np2 + 2 * fi + 1, pe), Bs,
-                              k0, nrows, lo, cnt);
+                              k0, nrows, lo, xs, xe);
     }
 }
 
@@ -514,16 +590,19 @@ __global__ __launch_bounds__(NW * kWave) void k_p2_conv(EngineArgs a, int nf, in
 // registers, inverse x-FFT, crop -> X2 plane planeOut.
 // ---------------------------------------------------------------------------
 template <int M, int RY, int NW, class PL, class S>
-__global__ __launch_bounds__(NW * kWave) void k_p2_corr(EngineArgs a, int planeIn, int planeOut,
-                                                        int tiles)
+__global__ __launch_bounds__(NW * kWave)
+__attribute__((amdgpu_waves_per_eu(p2_waves_per_eu<M, PL, kP2CorrPass>())))
+void k_p2_corr(EngineArgs a, int planeIn, int planeOut, int tiles)
 {
     constexpr int V = M / kWave;
     constexpr int H = M / 2 + 1;
     constexpr int HP = spec_pitch(M);
     constexpr int LP = p2_pitch<M, RY>();
+    constexpr bool ONE = p2_one_line<M, PL, kP2CorrPass>();
+    static_assert(!ONE || NW == RY, "one band row per wavefront");
     extern __shared__ float2 sm[];
     float2* A = sm;
-    float2* Bs = sm + RY * LP;
+    float2* Bs = ONE ? sm : sm + RY * LP;
     const int lane = lane_id(), w = wave_id();
     PL FP;
     FP.load(a.tw, lane);
@@ -539,10 +618,47 @@ __global__ __launch_bounds__(NW * kWave) void k_p2_corr(EngineArgs a, int planeI
         items.get(k, slot, band, fi);
         const int k0 = band * RY;
         const int nrows = items.rows(band);
+        const Tile t = a.tiles[slot];
+        const int lo = t.ox0 - t.p0x - a.halo2;
+        const int cnt = t.ox1 - t.ox0 + 2 * a.halo2;
+        S* dst = plane_at<S>(a.X2, (long long)slot * a.np2 + planeOut, (long long)a.S2 * HP);
+        // the sources vanish off the valid region: P1A leaves columns there unwritten
+        const int sx0 = a.e - t.p0x, sx1 = a.N - a.e - t.p0x;
+        if constexpr (ONE) {
+            const bool row = w < nrows;
+            float2* la = A + w * LP;
+            BandRegs<M, RY, NW, S> pb;
+            {
+                BandRegs<M, RY, NW, S> pa;
+                pa.fetch(band_src(slot, band, 0), nrows, sx0, sx1);
+                if (k) __syncthreads();  // the previous item's store has read A
+                pa.put(A);
+            }
+            __syncthreads();
+            pb.fetch(band_src(slot, band, 1), nrows, sx0, sx1);  // in flight during the first transform
+            float2 X[V];
+            if (row) fft_lds_to_regs<M, -1>(FP, la, X, lane);
+            __syncthreads();  // every line is read: the lines take the second plane
+            pb.put(A);
+            __syncthreads();
+            if (row) {
+                const float2* gxr = a.GT + (long long)(k0 + w) * M + lane;
+                const float2* gyr = a.GT + (long long)(H + k0 + w) * M + lane;
+                float2 Y[V];
+                fft_lds_to_regs<M, -1>(FP, la, Y, lane);
+#pragma unroll
+                for (int i = 0; i < V; ++i)
+                    X[i] = cadd(cmulc(X[i], ldu(gxr, kWave * i)), cmulc(Y[i], ldu(gyr, kWave * i)));
+                fft_from_regs<M, +1>(FP, la, X, lane);
+            }
+            __syncthreads();
+            store_band<M, RY, NW>(dst, A, k0, nrows, lo, 0, cnt);
+            continue;
+        }
         {
             BandRegs<M, RY, NW, S> pa, pb;
-            pa.fetch(band_src(slot, band, 0), nrows);
-            pb.fetch(band_src(slot, band, 1), nrows);
+            pa.fetch(band_src(slot, band, 0), nrows, sx0, sx1);
+            pb.fetch(band_src(slot, band, 1), nrows, sx0, sx1);
             if (k) __syncthreads();  // the previous item's store has read A
             pa.put(A);
             pb.put(Bs);
@@ -567,12 +683,7 @@ __global__ __launch_bounds__(NW * kWave) void k_p2_corr(EngineArgs a, int planeI
             fft_from_regs<M, +1>(FP, la, X, lane);
         }
         __syncthreads();
-        const Tile t = a.tiles[slot];
-        const int lo = t.ox0 - t.p0x - a.halo2;
-        const int cnt = t.ox1 - t.ox0 + 2 * a.halo2;
-        store_band<M, RY, NW>(plane_at<S>(a.X2, (long long)slot * a.np2 + planeOut,
-                                          (long long)a.S2 * HP),
-                              A, k0, nrows, lo, cnt);
+        store_band<M, RY, NW>(dst, A, k0, nrows, lo, 0, cnt);
     }
 }
 
@@ -694,7 +805,12 @@ __attribute__((amdgpu_waves_per_eu(M <= 512 && NF == 1 && !IsMfmaPlan<PL>::value
                                  (long long)xi2 * HP
                            : reinterpret_cast<const T*>(a.zcol);
     };
-    auto col = [&](int plane, int j) { return colS(float2{}, plane, j); };
+    // u planes: stored only for global columns [e, N - e) (u_columns)
+    auto col = [&](int plane, int j) {
+        const int c = t.ox0 + j;
+        return c >= a.e && c < a.N - a.e ? colS(float2{}, plane, j)
+                                         : static_cast<const float2*>(a.zcol);
+    };
     const long long ib = (long long)t.b * a.img_stride;
 
     if (with_u) {
@@ -718,16 +834,26 @@ __attribute__((amdgpu_waves_per_eu(M <= 512 && NF == 1 && !IsMfmaPlan<PL>::value
             const int px = 2 * fi, py = 2 * fi + 1;
             float2 u[V];
             float wy0[V], cv[V];
-            // own columns: u_x + i u_y, w_x to LDS, w_y kept
+            // own columns: u_x + i u_y, w_x to LDS, w_y kept; off the valid
+            // region the cotangent is zero and the transform is skipped
+            auto u_pair = [&](int j) {
+                const int c = t.ox0 + j;
+                if (c >= e && c < N - e) {
+                    ifft_pair<M, PL>(FP, line, col(px, j), col(py, j), u, lane);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < V; ++q) u[q] = make_float2(0.f, 0.f);
+                }
+            };
             cot(gx, cv);
-            ifft_pair<M, PL>(FP, line, col(px, j0), col(py, j0), u, lane);
+            u_pair(j0);
 #pragma unroll
             for (int j = 0; j < V; ++j) {
                 wx[(2 * w + 1) * M + lane + kWave * j] = u[j].x * cv[j];
                 wy0[j] = u[j].y * cv[j];
             }
             cot(gx + 1, cv);
-            ifft_pair<M, PL>(FP, line, col(px, j0 + 1), col(py, j0 + 1), u, lane);
+            u_pair(j0 + 1);
 #pragma unroll
             for (int j = 0; j < V; ++j) {
                 wx[(2 * w + 2) * M + lane + kWave * j] = u[j].x * cv[j];
@@ -746,9 +872,15 @@ __attribute__((amdgpu_waves_per_eu(M <= 512 && NF == 1 && !IsMfmaPlan<PL>::value
             if (w == P3A_WAVES - 1) {
                 // halo columns jb - 1 and jb + CBK: u_x only
                 float cr[V];
-                cot(t.ox0 + jb - 1, cv);
-                cot(t.ox0 + jb + CBK, cr);
-                ifft_pair<M, PL>(FP, line, col(px, jb - 1), col(px, jb + CBK), u, lane);
+                const int cl = t.ox0 + jb - 1, cr0 = t.ox0 + jb + CBK;
+                cot(cl, cv);
+                cot(cr0, cr);
+                if ((cl >= e && cl < N - e) || (cr0 >= e && cr0 < N - e)) {
+                    ifft_pair<M, PL>(FP, line, col(px, jb - 1), col(px, jb + CBK), u, lane);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < V; ++q) u[q] = make_float2(0.f, 0.f);
+                }
 #pragma unroll
                 for (int j = 0; j < V; ++j) {
                     wx[lane + kWave * j] = u[j].x * cv[j];
@@ -865,10 +997,10 @@ struct P2Waves {
     static constexpr int NW = (IsMfmaPlan<PL>::value && M >= 512 && RY >= 8) ? RY / 2 : RY;
 };
 
-template <int M, int RY>
+template <int M, int RY, class PL, int PASS>
 constexpr size_t p2_lds()
 {
-    return 2ull * RY * p2_pitch<M, RY>() * sizeof(float2);
+    return (p2_one_line<M, PL, PASS>() ? 1ull : 2ull) * RY * p2_pitch<M, RY>() * sizeof(float2);
 }
 
 template <int M, int RY, class PL, class SX>
@@ -878,13 +1010,15 @@ hipError_t prepare_kernels()
     static hipError_t st = hipSuccess;
     if (done) return st;
     constexpr int NW = P2Waves<M, RY, PL>::NW;
-    const void* fns[] = {reinterpret_cast<const void*>(&k_p2_conv<M, RY, NW, PL, float2>),
-                         reinterpret_cast<const void*>(&k_p2_conv<M, RY, NW, PL, SX>),
-                         reinterpret_cast<const void*>(&k_p2_corr<M, RY, NW, PL, SX>)};
-    for (const void* fn : fns)
+    constexpr int lc = (int)p2_lds<M, RY, PL, kP2ConvPass>();
+    constexpr int lr = (int)p2_lds<M, RY, PL, kP2CorrPass>();
+    const struct { const void* fn; int lds; } fns[] = {
+        {reinterpret_cast<const void*>(&k_p2_conv<M, RY, NW, PL, float2>), lc},
+        {reinterpret_cast<const void*>(&k_p2_conv<M, RY, NW, PL, SX>), lc},
+        {reinterpret_cast<const void*>(&k_p2_corr<M, RY, NW, PL, SX>), lr}};
+    for (const auto& f : fns)
         if (st == hipSuccess)
-            st = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)p2_lds<M, RY>());
+            st = hipFuncSetAttribute(f.fn, hipFuncAttributeMaxDynamicSharedMemorySize, f.lds);
     done = true;
     return st;
 }
@@ -900,7 +1034,8 @@ int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s0,
     constexpr int NW = P2Waves<M, RY, PL>::NW;
     hipError_t st = prepare_kernels<M, RY, PL, SX>();
     if (st != hipSuccess) return (int)st;
-    const size_t lds2 = p2_lds<M, RY>();
+    const size_t lds2c = p2_lds<M, RY, PL, kP2ConvPass>();
+    const size_t lds2r = p2_lds<M, RY, PL, kP2CorrPass>();
     const int cols_out = a0.S2 - 2 * a0.halo2;  // max output columns per tile
     bool aux_used = false;
     for (int t0 = 0, ci = 0; t0 < ntiles; t0 += chunk, ++ci) {
@@ -955,18 +1090,18 @@ int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s0,
             });
         };
         auto p2c = [&](int nf, bool x16, hipStream_t ls) {
-            const dim3 g2(P2Items<M, RY, P2C_ITEMS>::blocks(c, nf));
+            const dim3 g2(P2Items<M, RY, p2c_items<M, PL>()>::blocks(c, nf));
             run(kP2Conv, ls, [&] {
                 if (x16)
-                    k_p2_conv<M, RY, NW, PL, SX><<<g2, b2, lds2, ls>>>(a, nf, c);
+                    k_p2_conv<M, RY, NW, PL, SX><<<g2, b2, lds2c, ls>>>(a, nf, c);
                 else
-                    k_p2_conv<M, RY, NW, PL, float2><<<g2, b2, lds2, ls>>>(a, nf, c);
+                    k_p2_conv<M, RY, NW, PL, float2><<<g2, b2, lds2c, ls>>>(a, nf, c);
             });
         };
         auto p2r = [&](int in, int out, hipStream_t ls) {
             const dim3 g2(P2Items<M, RY, 1>::blocks(c, 1));
             run(kP2Corr, ls,
-                [&] { k_p2_corr<M, RY, NW, PL, SX><<<g2, b2, lds2, ls>>>(a, in, out, c); });
+                [&] { k_p2_corr<M, RY, NW, PL, SX><<<g2, b2, lds2r, ls>>>(a, in, out, c); });
         };
         auto p3f = [&](int nf) {
             run(kP3Fwd, s, [&] {

@@ -256,3 +256,48 @@ def test_ragged_groups_in_one_graph(orc):
         for got, want in ((s["tf"], a1 + a2), (s["tg"], b1 + b2)):
             scale = float(want.abs().max())
             assert float((got - want).abs().max()) <= 1e-5 * scale + 1e-6
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16_exchange"])
+def test_large_image_1024_windows(orc, prec):
+    """N = 16384, KW = 121, M = 1024: one 1024 x 1024 window of h (across
+    forward tile seams) and one of lambda_f, lambda_g (across adjoint tile
+    seams) against the FP64 oracle's FFT form on the sub-image that holds
+    every tap the window reads, at the tolerance of the form."""
+    torch = _torch()
+    tol_rel, tol_abs = (1e-3, 1e-5) if prec == "fp32" else (1e-2, 1e-3)
+    N, kw, M, w = 16384, 121, 1024, 1024
+    e = (kw - 1) // 2
+    W = N - 2 * e
+    gx, gy, hx, hy = srk.kernel_spectra_srsrk(kw, e / 3.0, M, with_spatial=True)
+    Kx, Ky = hx.numpy().astype(np.float64), hy.numpy().astype(np.float64)
+    f = patterns.grf_torch(1, N, seed=3100)
+    g = patterns.grf_torch(1, N, seed=4100)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(12)
+    v = torch.randn((1, W, W), generator=gen, device="cuda")
+    ws = srk.Workspace(N, W, e, M, precision=prec)
+    h = srk.forward_batched(g, f, gx, gy, e, 1.0, ws)
+    lf, lg = srk.adjoint_batched(v, g, f, gx, gy, e, 1.0, ws)
+    torch.cuda.synchronize()
+
+    def close(a, n, what):
+        err = np.abs(a - n)
+        lim = np.maximum(tol_abs, tol_rel * np.maximum(np.abs(a), np.abs(n)))
+        assert not (err > lim).any(), (
+            f"{prec} {what}: {(err > lim).sum()} violations, max err {err.max():.3e}, "
+            f"max|ref| {np.abs(n).max():.3e}")
+
+    ox0 = oy0 = 3 * (M - 2 * e) - w // 2   # straddles forward tile seams
+    S0 = w + 2 * e + 2
+    ref = orc.forward(crop(f[0], ox0, oy0, S0), crop(g[0], ox0, oy0, S0), e, Kx, Ky, fft=True)
+    close(h[0, ox0:ox0 + w, oy0:oy0 + w].double().cpu().numpy(), ref[:w, :w], "h")
+    pad = 2 * e + 2
+    qx0, qy0 = 5 * (M - 2 * e - 2) - w // 2, N // 2   # straddles adjoint tile seams
+    X0, Y0, S = qx0 - pad, qy0 - pad, w + 2 * pad
+    ra, rb = orc.adjoint(crop(v[0], X0, Y0, S - 2 * e), crop(f[0], X0, Y0, S),
+                         crop(g[0], X0, Y0, S), e, Kx, Ky, fft=True)
+    close(lf[0, qx0:qx0 + w, qy0:qy0 + w].double().cpu().numpy(),
+          ra[pad:pad + w, pad:pad + w], "lambda_f")
+    close(lg[0, qx0:qx0 + w, qy0:qy0 + w].double().cpu().numpy(),
+          rb[pad:pad + w, pad:pad + w], "lambda_g")
