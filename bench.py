@@ -46,6 +46,30 @@ def alg_bytes(N, W, B, e_eq_i):
     return per * B
 
 
+def pass_bytes_per_tile(N, M, e):
+    """Compulsory HBM bytes of each pass for one whole-image tile (N = M, E = I,
+    FP32 planes): what the pass must read and write given the three-pass
+    design (input images, the exchange planes it consumes and produces,
+    outputs), the algorithmic figure its per-launch roofline is priced on.
+    Planes hold H = M/2 + 1 complex (8-byte) rows per column; u is stored
+    and read on the valid columns only, the adjoint sources likewise."""
+    if N != M:
+        return None
+    W, H = N - 2 * e, M // 2 + 1
+    img, val = 4 * N * N, 4 * W * W
+    x1 = 8 * M * H                 # one X1 plane, all columns
+    col = 8 * H                    # one X2 / X1 column
+    src_cols = 4 * sum(1 for b in range(M // 4) if 4 * b + 3 >= e and 4 * b < N - e)
+    return {
+        "k_p1_cols": img + x1,
+        "k_p1_adj_src": img + val + 2 * col * src_cols,
+        "k_p2_conv": x1 + 2 * col * W,
+        "k_p2_corr": 2 * col * W + col * (N + 2),
+        "k_p3_fwd": 2 * col * W + 4 * N * (W + 2) + val,
+        "k_p3_adj": 2 * col * W + col * (N + 2) + val + 2 * img,
+    }
+
+
 def cpu_cores():
     try:
         return len(os.sched_getaffinity(0))
@@ -243,6 +267,14 @@ def main():
     else:
         ms, t_f, t_a, ms_eager, kernels = measure("fp32")
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"]) if kernels else None
+    pbt = pass_bytes_per_tile(N, M, e)
+    if pbt:
+        for name, kv in kernels.items():
+            if name in pbt:
+                per_launch = pbt[name] * kv["tiles_per_launch"]
+                kv["pass_bytes_per_launch"] = int(per_launch)
+                kv["pass_GBps"] = round(per_launch / (kv["avg_launch_us"] * 1e3), 1)
+                kv["pass_frac"] = round(kv["pass_GBps"] / HBM_PEAK_GBS, 4)
     for prec, r in reduced.items():
         r["speedup_vs_fp32"] = round(ms / r["ms_per_step"], 3)
         tr = pmc_traffic(N, B, kw, M, prec)
@@ -399,7 +431,16 @@ def main():
                               "passes); algorithmic bytes 16N^2+8W^2 per image = "
                               f"{nbytes / 1e9:.2f} GB per step; traffic = HBM bytes per step "
                               "from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary in profiles/",
-                     "dominant_kernel": dom},
+                     "dominant_kernel": dom,
+                     "dominant_kernel_pass": (
+                         {"kernel": dom, "bytes_per_launch": kernels[dom]["pass_bytes_per_launch"],
+                          "avg_launch_us": kernels[dom]["avg_launch_us"],
+                          "achieved": kernels[dom]["pass_GBps"], "unit": "GB/s",
+                          "frac": kernels[dom]["pass_frac"],
+                          "note": "the pass's own compulsory bytes (its input planes and "
+                                  "images, its output planes) per launch / its average "
+                                  "launch duration; launches of the two lanes overlap"}
+                         if dom and "pass_GBps" in kernels.get(dom, {}) else None)},
         "ms_per_step_eager_timed": round(ms_eager, 3) if ms_eager else None,
         "kernels": kernels,
         "mixed_fp16": reduced.get("mixed_fp16"),
