@@ -23,12 +23,16 @@ def instances(count: int = 64, seed: int = 4242, sizes: Sequence[int] = (1024, 2
     return [(int(rng.choice(sizes)), int(rng.choice(widths))) for _ in range(count)]
 
 
-def tile_for(N: int, kw: int) -> int:
+def tile_for(N: int, kw: int, precision: str = "fp32") -> int:
     """Tile size for a group: one whole-image tile up to N = 1024, else
-    1024-point overlap-and-save tiles (2048 if the halo leaves no room)."""
+    1024-point overlap-and-save tiles (2048 if the halo leaves no room).  The
+    FP16-MFMA form runs faster on 512-point tiles beyond N = 512 (its
+    1024-point transforms hold too many registers)."""
+    e = (kw - 1) // 2
+    if precision == "mixed_fp16" and N > 512 and 512 > 2 * e + 2:
+        return 512
     if N <= 1024:
         return next(M for M in srk.SUPPORTED_M if M >= N)
-    e = (kw - 1) // 2
     return 1024 if 1024 > 2 * e + 2 else 2048
 
 
@@ -50,7 +54,7 @@ class RaggedBatch:
             e = (kw - 1) // 2
             W = N - 2 * e
             B = len(idx)
-            M = tile_for(N, kw)
+            M = tile_for(N, kw, precision)
             gx, gy = srk.kernel_spectra_srsrk(kw, e / 3.0, M)
             ws = srk.Workspace(N, W, e, M, B_max=B, precision=precision, scratch_mb=1024)
             s = dict(N=N, kw=kw, e=e, W=W, B=B, M=M, idx=idx, gx=gx, gy=gy, ws=ws,
@@ -74,8 +78,18 @@ class RaggedBatch:
         return sum(N * N for N, _ in self.inst)
 
     def set_precision(self, precision: str) -> None:
+        """Switch every group to a precision form, rebuilding the workspace and
+        spectra of groups whose tile size differs for that form."""
         for s in self.groups:
+            M = tile_for(s["N"], s["kw"], precision)
+            if M != s["M"]:
+                s["ws"].close()
+                s["gx"], s["gy"] = srk.kernel_spectra_srsrk(s["kw"], s["e"] / 3.0, M)
+                s["ws"] = srk.Workspace(s["N"], s["W"], s["e"], M, B_max=s["B"],
+                                        precision=precision, scratch_mb=1024)
+                s["M"] = M
             s["ws"].set_precision(precision)
+        self.graph = None
 
     def run(self) -> None:
         for s in self.groups:
