@@ -1,0 +1,108 @@
+"""GPU parity over the batched training configurations and over odd
+geometries, against the FP64 oracle at the fp32 tolerance (README.md:8-10).
+
+Batched training forward (srkTrainingForward): B = 3, N in {256, 512},
+KW in {31, 61, 121}, dx in {4, 10} (sigma 20 = 1 / gamma for gamma = 0.05),
+E = I and E != I Gaussian random fields.  Odd geometries: N not a power of
+two, the smallest kernels (KW = 3, 5), overlap-and-save tiles whose last
+row and column of tiles are partial, forward, adjoint, tangent and
+tangent-adjoint.
+"""
+import numpy as np
+import pytest
+
+import shrinkage_full_tests_amd as srk
+from shrinkage_full_tests_amd import patterns
+
+pytestmark = pytest.mark.gpu
+
+TOL_REL, TOL_ABS = 1e-3, 1e-5
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+
+
+def host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy().astype(np.float64)
+
+
+def assert_close(a, n, what=""):
+    err = np.abs(a - n)
+    lim = np.maximum(TOL_ABS, TOL_REL * np.maximum(np.abs(a), np.abs(n)))
+    bad = err > lim
+    assert not bad.any(), (f"{what}: {bad.sum()} violations, max err {err.max():.3e}, "
+                           f"max|ref| {np.abs(n).max():.3e}")
+
+
+def kernels(kw, sigma, M, dx=1.0):
+    gx, gy, hx, hy = srk.kernel_spectra_srsrk(kw, sigma, M, dx=dx, with_spatial=True)
+    return gx, gy, hx.numpy().astype(np.float64), hy.numpy().astype(np.float64)
+
+
+@pytest.mark.parametrize("dx", [4.0, 10.0])
+@pytest.mark.parametrize("kw", [31, 61, 121])
+@pytest.mark.parametrize("N", [256, 512])
+def test_training_forward_configs(orc, N, kw, dx):
+    e = (kw - 1) // 2
+    W = N - 2 * e
+    B = 3
+    gx, gy, Kx, Ky = kernels(kw, 20.0, N, dx)
+    E = patterns.grf(N, 300 + kw, B=B)
+    I = patterns.grf(N, 400 + kw, B=B)
+    ws = srk.Workspace(N, W, e, N, B_max=B)
+    R = dev(np.zeros((B, W, W)))
+    dE, dI = dev(E), dev(I)
+    s = dx * dx
+    srk.srkTrainingForward(dE, dE, gx, gy, N, W, e, dx, B, ws, R)
+    Rh = host(R)
+    for b in range(B):
+        assert_close(Rh[b], orc.forward(E[b], E[b], e, Kx, Ky, s=s, fft=True), f"E=I b{b}")
+    srk.srkTrainingForward(dE, dI, gx, gy, N, W, e, dx, B, ws, R)
+    Rh = host(R)
+    for b in range(B):
+        assert_close(Rh[b], orc.forward(I[b], E[b], e, Kx, Ky, s=s, fft=True), f"E!=I b{b}")
+
+
+@pytest.mark.parametrize("N,kw,M", [(257, 3, 512), (301, 5, 512), (300, 31, 512),
+                                    (777, 61, 256), (517, 121, 256)])
+def test_odd_geometry_forward_adjoint(orc, N, kw, M):
+    e = (kw - 1) // 2
+    W = N - 2 * e
+    sigma = max(e / 3.0, 1.0)
+    gx, gy, Kx, Ky = kernels(kw, sigma, M)
+    f = patterns.grf(N, 500 + N)[0]
+    g = patterns.grf(N, 600 + N)[0]
+    v = np.random.default_rng(N).standard_normal((W, W)).astype(np.float32)
+    ws = srk.Workspace(N, W, e, M)
+    h = srk.forward_batched(dev(g[None]), dev(f[None]), gx, gy, e, 1.0, ws)
+    assert_close(host(h)[0], orc.forward(f, g, e, Kx, Ky, fft=True), "h")
+    lf, lg = srk.adjoint_batched(dev(v[None]), dev(g[None]), dev(f[None]), gx, gy, e, 1.0, ws)
+    rf, rg = orc.adjoint(v, f, g, e, Kx, Ky, fft=True)
+    assert_close(host(lf)[0], rf, "lambda_f")
+    assert_close(host(lg)[0], rg, "lambda_g")
+
+
+@pytest.mark.parametrize("N,kw,M", [(301, 5, 512), (517, 31, 256)])
+def test_odd_geometry_tangent_pair(orc, N, kw, M):
+    e = (kw - 1) // 2
+    W = N - 2 * e
+    sigma = max(e / 3.0, 1.0)
+    gx, gy, Kx, Ky = kernels(kw, sigma, M)
+    f, g, df, dg = (patterns.grf(N, s)[0] for s in (700 + N, 800 + N, 900 + N, 1000 + N))
+    rng = np.random.default_rng(N + 1)
+    lam = rng.standard_normal((W, W)).astype(np.float32)
+    dlam = rng.standard_normal((W, W)).astype(np.float32)
+    ws = srk.Workspace(N, W, e, M)
+    dh = dev(np.zeros((W, W)))
+    srk.shrinkageTangentOaS(dev(f), dev(g), dev(df), dev(dg), gx, gy, N, W, e, 1.0, ws, dh)
+    assert_close(host(dh), orc.tangent(f, g, df, dg, e, Kx, Ky), "dh")
+    a, b = dev(np.zeros((N, N))), dev(np.zeros((N, N)))
+    srk.shrinkageTangentAdjointOaS(dev(f), dev(g), dev(df), dev(dg), dev(lam), dev(dlam), gx, gy,
+                                   N, W, e, 1.0, ws, a, b)
+    ra, rb = orc.tangent_adjoint(f, g, df, dg, lam, dlam, e, Kx, Ky)
+    assert_close(host(a), ra, "dlam_f")
+    assert_close(host(b), rb, "dlam_g")
