@@ -204,8 +204,8 @@ __global__ __launch_bounds__(256) void k_p1_cols(EngineArgs a)
             for (int i = 0; i < KV; ++i) {
                 const int k = lane + kWave * i;
                 if (k < H) {
-                    const float2 z = line[lslot(k)];
-                    const float2 zm = line[lslot((M - k) & (M - 1))];
+                    const float2 z = line[lslot<M>(k)];
+                    const float2 zm = line[lslot<M>((M - k) & (M - 1))];
                     P[i] = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
                     Q[i] = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
                 }
@@ -297,8 +297,8 @@ __global__ __launch_bounds__(256) void k_p1_adj_src(EngineArgs a, int plane0)
         for (int i = 0; i < KV; ++i) {
             const int k = lane + kWave * i;
             if (k < H) {
-                const float2 z = line[lslot(k)];
-                const float2 zm = line[lslot((M - k) & (M - 1))];
+                const float2 z = line[lslot<M>(k)];
+                const float2 zm = line[lslot<M>((M - k) & (M - 1))];
                 P[i] = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
                 Q[i] = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
             }
@@ -387,11 +387,14 @@ struct P2Items {
     __device__ int rows(int band) const { return band < NBF ? RY : 1; }
 };
 
-// LDS pitch of the P2 band lines: line_slots(M) + 32/RY, so that the
-// transposing accesses below (lane -> (row lane % RY, column lane / RY)) fall
-// on distinct banks within each half wavefront.
+// LDS pitch of the P2 band lines: line_slots(M) + 16 - 16/RY, so that the
+// transposing accesses below (lane -> (row lane % RY, column lane / RY)) are
+// conflict-free for the band writes (32 banks, 16 lanes a cycle) and leave
+// the store's reads (64 banks, 32 lanes a cycle) an eighth of their lanes in
+// conflict (a pitch of line_slots + 32/RY did the opposite: the writes of
+// every 16-lane group hit 8 banks twice).
 template <int M, int RY>
-constexpr int p2_pitch() { return line_slots(M) + 32 / RY; }
+constexpr int p2_pitch() { return line_slots(M) + 16 - 16 / RY; }
 
 // Band of one X1 plane (M x RY elements, contiguous) -> registers of the
 // block's threads (fetch_band) -> LDS lines (put_band): line r holds k_y row
@@ -420,9 +423,9 @@ struct BandRegs {
     __device__ __forceinline__ void put(float2* L) const
     {
         constexpr int XS = T / RY, LP = p2_pitch<M, RY>();
-        float2* d = L + (threadIdx.x % RY) * LP + lslot(threadIdx.x / RY);
+        float2* d = L + (threadIdx.x % RY) * LP + lslot<M>(threadIdx.x / RY);
 #pragma unroll
-        for (int i = 0; i < IT; ++i) d[i * slot_off(XS)] = unpack(v[i]);
+        for (int i = 0; i < IT; ++i) d[i * slot_off<M>(XS)] = unpack(v[i]);
     }
 };
 
@@ -441,7 +444,7 @@ __device__ __forceinline__ void store_band(S* __restrict__ plane, const float2* 
     for (int xi = xi0; xi < xe; xi += XS, d += (long long)XS * HP) {
         const int x = lo + xi;
         float2 v = make_float2(0.f, 0.f);
-        if (x >= 0 && x < M) v = src[lslot(x)];
+        if (x >= 0 && x < M) v = src[lslot<M>(x)];
         *d = pack<S>(v);
     }
 }
@@ -859,13 +862,13 @@ __attribute__((amdgpu_waves_per_eu(M <= 512 && NF == 1 && !IsMfmaPlan<PL>::value
                 wx[(2 * w + 2) * M + lane + kWave * j] = u[j].x * cv[j];
                 // w_y of both columns in the line; the y-stencil transpose reads
                 // the neighbouring rows (same wavefront, in order)
-                line[lslot(lane + kWave * j)] = make_float2(wy0[j], u[j].y * cv[j]);
+                line[lslot<M>(lane + kWave * j)] = make_float2(wy0[j], u[j].y * cv[j]);
             }
 #pragma unroll
             for (int j = 0; j < V; ++j) {
                 const int y = lane + kWave * j;
-                const float2 wm = y - 1 >= 0 ? line[lslot(y - 1)] : make_float2(0.f, 0.f);
-                const float2 wp = y + 1 < M ? line[lslot(y + 1)] : make_float2(0.f, 0.f);
+                const float2 wm = y - 1 >= 0 ? line[lslot<M>(y - 1)] : make_float2(0.f, 0.f);
+                const float2 wp = y + 1 < M ? line[lslot<M>(y + 1)] : make_float2(0.f, 0.f);
                 acc0[j] += 0.5f * (wm.x - wp.x);
                 acc1[j] += 0.5f * (wm.y - wp.y);
             }

@@ -29,9 +29,17 @@ namespace srk {
 
 constexpr int kWave = 64;
 
+// Padded layout of an LDS line of M elements: one pad slot after every 2^S
+// elements, S = pad_shift<M>().  Measured LDS banking (reads: 64 four-byte
+// banks, 32 lanes a cycle; writes: 32 banks, 16 lanes a cycle) favours a pad
+// every 8 elements for the 512-point plan (8^3) and every 16 for the
+// 1024- and 2048-point plans, whose radix-16 first pass writes at stride 16.
+template <int M>
+__host__ __device__ constexpr int pad_shift() { return M >= 1024 ? 4 : 3; }
 // slot of element i in an LDS line, and slots per line of M elements
-__host__ __device__ constexpr int lslot(int i) { return i + (i >> 3); }
-__host__ __device__ constexpr int line_slots(int M) { return M + M / 8; }
+template <int M>
+__host__ __device__ constexpr int lslot(int i) { return i + (i >> pad_shift<M>()); }
+__host__ __device__ constexpr int line_slots(int M) { return M >= 1024 ? M + M / 16 : M + M / 8; }
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 {
@@ -160,45 +168,30 @@ struct PassTw {
     }
 };
 
-// Slot arithmetic of the padded layout.  For a multiple of eight C,
-// lslot(a + C) = lslot(a) + C + C/8, so every access of a pass is a per-lane
-// base slot plus a compile-time offset (immediate LDS offsets, no per-access
-// address arithmetic).
-__host__ __device__ constexpr int slot_off(int C) { return C + C / 8; }
+// Slot arithmetic of the padded layout.  For C a multiple of the pad period
+// P = 2^S, lslot(a + C) = lslot(a) + C + C/P, so every access of a pass is a
+// per-lane base slot plus a compile-time offset (immediate LDS offsets, no
+// per-access address arithmetic).
+template <int M>
+__host__ __device__ constexpr int slot_off(int C) { return C + (C >> pad_shift<M>()); }
 
 // Base slot and constant offset of output r of butterfly j in a pass of radix R
-// and span Ns (element index (j/Ns)*Ns*R + j%Ns + r*Ns).
-template <int R, int Ns>
+// and span Ns: element e = A + B + r Ns with A = (j/Ns) Ns R, B = j mod Ns.
+// For Ns a multiple of P, lslot(e) = lslot(A + B) + slot_off(r Ns).  For
+// Ns < P (Ns divides P), A + B + r Ns never crosses a pad boundary beyond the
+// ones r Ns crosses by itself: lslot(e) = A + A/P + B + slot_off(r Ns).
+template <int M, int R, int Ns>
 struct WriteSlots {
+    static constexpr int S = pad_shift<M>();
     __device__ static __forceinline__ int base(int j)
     {
-        const int e = (j / Ns) * Ns * R + (j & (Ns - 1));
-        if constexpr (Ns % 8 == 0) {
-            return lslot(e);
-        } else if constexpr (Ns == 1 && R % 8 == 0) {
-            return slot_off(R * j);
-        } else if constexpr (Ns == 1 && R == 4) {
-            return 4 * j + (j >> 1);         // (4j + r) >> 3 = j >> 1 for r < 4
-        } else if constexpr (Ns == 4 && R == 4) {
-            return e + 2 * (j / 4);          // e = 16 (j/4) + j%4
-        } else {
-            return e;                        // generic: offset() adds the padding
-        }
+        const int A = (j / Ns) * Ns * R, B = j & (Ns - 1);
+        if constexpr (Ns % (1 << S) == 0)
+            return lslot<M>(A + B);
+        else
+            return A + (A >> S) + B;
     }
-    __device__ static __forceinline__ int offset(int r, int e_unused)
-    {
-        if constexpr (Ns % 8 == 0) {
-            return slot_off(r * Ns);
-        } else if constexpr (Ns == 1 && R % 8 == 0) {
-            return r + (r >> 3);
-        } else if constexpr (Ns == 1 && R == 4) {
-            return r;
-        } else if constexpr (Ns == 4 && R == 4) {
-            return 4 * r + (r >= 2 ? 1 : 0);
-        } else {
-            return lslot(e_unused + r * Ns) - e_unused;
-        }
-    }
+    __device__ static __forceinline__ int offset(int r) { return slot_off<M>(r * Ns); }
 };
 
 // One Stockham pass of radix R with span Ns over a line of M values.
@@ -221,11 +214,11 @@ __device__ __forceinline__ void stockham_pass(float2* __restrict__ line, const f
 #pragma unroll
             for (int r = 0; r < R; ++r) x[g][r] = v[g + r * G];
     } else {
-        const float2* rb = line + lslot(lane);
+        const float2* rb = line + lslot<M>(lane);
 #pragma unroll
         for (int g = 0; g < G; ++g)
 #pragma unroll
-            for (int r = 0; r < R; ++r) x[g][r] = rb[slot_off(kWave * g + r * NB)];
+            for (int r = 0; r < R; ++r) x[g][r] = rb[slot_off<M>(kWave * g + r * NB)];
     }
 #pragma unroll
     for (int g = 0; g < G; ++g) {
@@ -243,10 +236,9 @@ __device__ __forceinline__ void stockham_pass(float2* __restrict__ line, const f
 #pragma unroll
             for (int r = 0; r < R; ++r) out[g + r * G] = x[g][r];
         } else {
-            const int e0 = (j / Ns) * Ns * R + (j & (Ns - 1));
-            float2* wb = line + WriteSlots<R, Ns>::base(j);
+            float2* wb = line + WriteSlots<M, R, Ns>::base(j);
 #pragma unroll
-            for (int r = 0; r < R; ++r) wb[WriteSlots<R, Ns>::offset(r, e0)] = x[g][r];
+            for (int r = 0; r < R; ++r) wb[WriteSlots<M, R, Ns>::offset(r)] = x[g][r];
         }
     }
 }

@@ -72,7 +72,7 @@ __device__ __forceinline__ void mfma16_pass(float2* __restrict__ line, const Dft
         for (int e = 0; e < 8; ++e) {
             const int j = 16 * rb + i;
             const int r = 8 * (q & 1) + e;
-            const float2 x = line[lslot(j + r * NB)];
+            const float2 x = line[lslot<M>(j + r * NB)];
             const float v = im_part ? x.y : x.x;
             const _Float16 hi = (_Float16)v;
             Ahi[rb][e] = hi;
@@ -102,7 +102,7 @@ __device__ __forceinline__ void mfma16_pass(float2* __restrict__ line, const Dft
             } else {
                 y = cmul(y, tw_next[rb * 4 + e]);
             }
-            line[lslot(o)] = y;
+            line[lslot<M>(o)] = y;
         }
 }
 
@@ -156,7 +156,7 @@ struct FftPlanMfma {
         run_lds<DIR, REGS>(line, v, lane);
         if constexpr (OREGS) {
 #pragma unroll
-            for (int k = 0; k < V; ++k) out[k] = line[lslot(lane + kWave * k)];
+            for (int k = 0; k < V; ++k) out[k] = line[lslot<M>(lane + kWave * k)];
         }
     }
 
@@ -166,7 +166,7 @@ struct FftPlanMfma {
         float2 x[V];
 #pragma unroll
         for (int k = 0; k < V; ++k) {
-            x[k] = REGS ? v[k] : line[lslot(lane + kWave * k)];
+            x[k] = REGS ? v[k] : line[lslot<M>(lane + kWave * k)];
             if (DIR > 0) x[k].y = -x[k].y;   // inverse = conj(DFT(conj(.)))
         }
         // power-of-two scale: operands of the last MFMA pass stay below 2^15
@@ -184,7 +184,7 @@ struct FftPlanMfma {
         if constexpr (R0 == 1) {
 #pragma unroll
             for (int k = 0; k < V; ++k)
-                line[lslot(lane + kWave * k)] = make_float2(x[k].x * sc, x[k].y * sc);
+                line[lslot<M>(lane + kWave * k)] = make_float2(x[k].x * sc, x[k].y * sc);
         } else {
             // radix-R0 Stockham pass (span 1) on registers: butterfly j = lane + 64 g
             // takes elements j + r * (M/R0) = x[g + r*G]
@@ -199,7 +199,7 @@ struct FftPlanMfma {
                 const int j = lane + kWave * g;
 #pragma unroll
                 for (int r = 0; r < R0; ++r)
-                    line[lslot(R0 * j + r)] = cmul(y[r], tw0[g * R0 + r]);
+                    line[lslot<M>(R0 * j + r)] = cmul(y[r], tw0[g * R0 + r]);
             }
         }
         mfma16_pass<M, R0, false, false>(line, T, twA, 1.0f, lane);
