@@ -128,6 +128,8 @@ def main():
                     help="tile size of the large image (0: time 512 and 1024, report the faster)")
     ap.add_argument("--no-ragged", action="store_true",
                     help="skip the ragged 64-instance fwd+adj+tangent-adjoint point")
+    ap.add_argument("--no-stored-u", action="store_true",
+                    help="skip the stored-activation (u kept by the forward) fwd+adj point")
     ap.add_argument("--no-refcfg", action="store_true",
                     help="skip the forward at the configuration of the reference's published timing")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
@@ -289,6 +291,47 @@ def main():
     achieved = nbytes / (ms / 1e3) / 1e9
     info = ws.info()
     traffic = pmc_traffic(N, B, kw, M, "fp32")
+
+    stored = None
+    if not args.no_stored_u:
+        # the reference's stored-activation flow, batched: the forward keeps
+        # u_x, u_y (8 W^2 bytes per image) and the adjoint reads them
+        ws.set_precision("fp32")
+        ux = torch.empty((B, W, W), device="cuda")
+        uy = torch.empty((B, W, W), device="cuda")
+
+        def step_u():
+            srk.forward_batched_u(E, E, gx, gy, e, dx, ws, R, ux, uy)
+            srk.adjoint_batched_u(v, ux, uy, E, gx, gy, e, dx, ws, lf, lg)
+
+        for _ in range(args.warmup):
+            step_u()
+        torch.cuda.synchronize()
+        gu = None
+        if not args.no_graph:
+            gu = torch.cuda.CUDAGraph()
+            su = torch.cuda.Stream()
+            su.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(su):
+                with torch.cuda.graph(gu, stream=su):
+                    step_u()
+            torch.cuda.current_stream().wait_stream(su)
+            gu.replay()
+            torch.cuda.synchronize()
+        run_u = gu.replay if gu is not None else step_u
+        start.record(stream)
+        for _ in range(args.steps):
+            run_u()
+        stop.record(stream)
+        torch.cuda.synchronize()
+        msu = start.elapsed_time(stop) / args.steps
+        stored = {"workload": "srkForwardBatchedU + srkAdjointBatchedU (u stored by the "
+                              "forward, as the reference's srkForwardOaS + adjointSRK), FP32",
+                  "value": round(B * N * N / (msu / 1e3) / 1e6, 1), "unit": "Mpix/s",
+                  "ms_per_step": round(msu, 3),
+                  "speedup_vs_recompute": round(ms / msu, 3),
+                  "stored_activation_bytes": 8 * B * W * W}
+        del gu, ux, uy
 
     secondary = None
     if not args.no_large:
@@ -461,6 +504,7 @@ def main():
         "bf16_exchange": reduced.get("bf16_exchange"),
         "secondary": secondary,
         "ragged": ragged_out,
+        "stored_u": stored,
         "reference_config_forward": refcfg,
         "cpu_baseline": cpu,
     }

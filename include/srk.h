@@ -227,6 +227,22 @@ int srkAdjointBatched(const float* v, const float* E, const float* I, const void
                       const void* Gy_freq, int N, int W, int erosion, float dx, int B,
                       hipStream_t stream, float* dLdI, float* dLdE, srk_workspace_t ws);
 
+/*
+ * The batched pair of the reference's stored-activation flow (srkForwardOaS
+ * with u outputs, srk_forward.cuh:468-501, then adjointSRK,
+ * adjoint/srk_adjoint.cuh:402-425): the forward also writes u_x, u_y =
+ * s conv(E, G) on the valid region [B x W x W] (FP32 in every precision), and
+ * the adjoint takes them instead of recomputing u.  Costs 8 B W^2 bytes of
+ * stored activations; saves the adjoint's convolution passes.
+ */
+int srkForwardBatchedU(const float* E, const float* I, const void* Gx_freq, const void* Gy_freq,
+                       int N, int W, int erosion, float dx, int B, hipStream_t stream, float* R,
+                       float* ux, float* uy, srk_workspace_t ws);
+int srkAdjointBatchedU(const float* v, const float* ux, const float* uy, const float* I,
+                       const void* Gx_freq, const void* Gy_freq, int N, int W, int erosion,
+                       float dx, int B, hipStream_t stream, float* dLdI, float* dLdE,
+                       srk_workspace_t ws);
+
 /* Batched tangent / tangent-adjoint ([B x ...] arrays, as above). */
 int srkTangentBatched(const float* f, const float* g, const float* df, const float* dg,
                       const void* Gx_freq, const void* Gy_freq, int N, int W, int erosion,

@@ -19,6 +19,7 @@ __all__ = [
     "precompute_kernel_freq",
     "srkTrainingForward", "srkForwardOaS", "adjointSRK", "shrinkageTangentOaS",
     "shrinkageTangentAdjointOaS", "forward_batched", "adjoint_batched", "tangent_batched",
+    "forward_batched_u", "adjoint_batched_u",
     "tangent_adjoint_batched", "ShrinkageFunction", "erosion_for", "SUPPORTED_M",
     "measure_cd_batched", "shrinkage_cd_batched",
 ]
@@ -231,6 +232,35 @@ def adjoint_batched(v, E, I, Gx_freq, Gy_freq, erosion, dx, ws, dLdI=None, dLdE=
         dLdE = torch.empty((B, N, N), dtype=torch.float32, device=E.device)
     call("srkAdjointBatched", _ptr(v), _ptr(E), _ptr(I), _ptr(Gx_freq), _ptr(Gy_freq), N, W,
          erosion, float(dx), B, _stream(stream), _ptr(dLdI), _ptr(dLdE), ws.handle)
+    return dLdI, dLdE
+
+
+def forward_batched_u(E, I, Gx_freq, Gy_freq, erosion, dx, ws, R=None, ux=None, uy=None,
+                      stream=None):
+    """Batched forward that also stores u_x, u_y [B, W, W] for adjoint_batched_u."""
+    torch = _torch()
+    B, N = E.shape[0], E.shape[-1]
+    W = N - 2 * erosion
+    R, ux, uy = (t if t is not None else torch.empty((B, W, W), dtype=torch.float32,
+                                                       device=E.device) for t in (R, ux, uy))
+    call("srkForwardBatchedU", _ptr(E), _ptr(I), _ptr(Gx_freq), _ptr(Gy_freq), N, W, erosion,
+         float(dx), B, _stream(stream), _ptr(R), _ptr(ux), _ptr(uy), ws.handle)
+    return R, ux, uy
+
+
+def adjoint_batched_u(v, ux, uy, I, Gx_freq, Gy_freq, erosion, dx, ws, dLdI=None, dLdE=None,
+                      stream=None):
+    """Batched adjoint from the u stored by forward_batched_u."""
+    torch = _torch()
+    B, N = I.shape[0], I.shape[-1]
+    W = N - 2 * erosion
+    if dLdI is None:
+        dLdI = torch.empty((B, N, N), dtype=torch.float32, device=I.device)
+    if dLdE is None:
+        dLdE = torch.empty((B, N, N), dtype=torch.float32, device=I.device)
+    call("srkAdjointBatchedU", _ptr(v), _ptr(ux), _ptr(uy), _ptr(I), _ptr(Gx_freq),
+         _ptr(Gy_freq), N, W, erosion, float(dx), B, _stream(stream), _ptr(dLdI), _ptr(dLdE),
+         ws.handle)
     return dLdI, dLdE
 
 

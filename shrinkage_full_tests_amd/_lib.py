@@ -42,6 +42,8 @@ SIGNATURES = {
                                         _p, _p, _p]),
     "srkForwardBatched": (_i, [_p, _p, _p, _p, _i, _i, _i, _f, _i, _p, _p, _p]),
     "srkAdjointBatched": (_i, [_p, _p, _p, _p, _p, _i, _i, _i, _f, _i, _p, _p, _p, _p]),
+    "srkForwardBatchedU": (_i, [_p, _p, _p, _p, _i, _i, _i, _f, _i, _p, _p, _p, _p, _p]),
+    "srkAdjointBatchedU": (_i, [_p, _p, _p, _p, _p, _p, _i, _i, _i, _f, _i, _p, _p, _p, _p]),
     "srkTangentBatched": (_i, [_p, _p, _p, _p, _p, _p, _i, _i, _i, _f, _i, _p, _p, _p]),
     "srkTangentAdjointBatched": (_i, [_p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _f, _i, _p,
                                       _p, _p, _p]),

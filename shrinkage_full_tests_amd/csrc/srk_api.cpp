@@ -700,6 +700,35 @@ int srkForwardBatched(const float* E, const float* I, const void* Gx, const void
     return run_fwd(w, srk::Op::Forward, a, B, stream);
 }
 
+int srkForwardBatchedU(const float* E, const float* I, const void* Gx, const void* Gy, int N,
+                       int W, int erosion, float dx, int B, hipStream_t stream, float* R,
+                       float* ux, float* uy, srk_workspace_t w)
+{
+    SRK_TRY(check_ws(w, N, W, erosion, B));
+    SRK_TRY(check_f(E)); SRK_TRY(check_f(I)); SRK_TRY(check_f(R)); SRK_TRY(check_spec(Gx, Gy));
+    SRK_TRY(check_f(ux)); SRK_TRY(check_f(uy));
+    SRK_TRY(import_spectra(w, Gx, Gy, dx, stream));
+    srk::EngineArgs a = base_args(w, dx);
+    a.g0 = E; a.f0 = I; a.out0 = R;
+    a.ux_out = ux; a.uy_out = uy;
+    return run_fwd(w, srk::Op::Forward, a, B, stream);
+}
+
+int srkAdjointBatchedU(const float* v, const float* ux, const float* uy, const float* I,
+                       const void* Gx, const void* Gy, int N, int W, int erosion, float dx, int B,
+                       hipStream_t stream, float* dLdI, float* dLdE, srk_workspace_t w)
+{
+    SRK_TRY(check_ws(w, N, W, erosion, B));
+    SRK_TRY(check_f(v)); SRK_TRY(check_f(ux)); SRK_TRY(check_f(uy)); SRK_TRY(check_f(I));
+    SRK_TRY(check_f(dLdI)); SRK_TRY(check_f(dLdE)); SRK_TRY(check_spec(Gx, Gy));
+    SRK_TRY(hip_status(srk::engine_adjoint_f_from_u(v, ux, uy, dLdI, N, W, erosion, B,
+                                                    (long long)N * N, (long long)W * W, stream)));
+    SRK_TRY(import_spectra(w, Gx, Gy, dx, stream));
+    srk::EngineArgs a = base_args(w, dx);
+    a.f0 = I; a.v0 = v; a.out1 = dLdE;
+    return run_adj(w, srk::Op::AdjointG, a, B, stream);
+}
+
 int srkTrainingForward(const float* E, const float* I, const void* Gx, const void* Gy, int N,
                        int W, int erosion, float dx, int B, hipStream_t stream, float* R,
                        srk_workspace_t w)
