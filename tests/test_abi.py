@@ -67,6 +67,8 @@ def test_entry_points_reject_null_workspace():
     assert L.srkForwardBatched(1, 1, 8, 8, 256, 226, 15, 1.0, 1, None, 1, None) == -3
     assert L.srkAdjointBatched(1, 1, 1, 8, 8, 256, 226, 15, 1.0, 1, None, 1, 1, None) == -3
     assert L.shrinkageTangentOaS(1, 1, 1, 1, 8, 8, 256, 226, 15, 1.0, None, 1, None) == -3
+    assert L.srkForwardBatchedU(1, 1, 8, 8, 256, 226, 15, 1.0, 1, None, 1, 1, 1, None) == -3
+    assert L.srkAdjointBatchedU(1, 1, 1, 1, 8, 8, 256, 226, 15, 1.0, 1, None, 1, 1, None) == -3
 
 
 def test_kernel_spectra_argument_errors():
