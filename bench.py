@@ -128,6 +128,8 @@ def main():
                     help="tile size of the large image (0: time 512 and 1024, report the faster)")
     ap.add_argument("--no-ragged", action="store_true",
                     help="skip the ragged 64-instance fwd+adj+tangent-adjoint point")
+    ap.add_argument("--f-ne-g", action="store_true",
+                    help="main point with distinct images for f (I) and g (E)")
     ap.add_argument("--no-stored-u", action="store_true",
                     help="skip the stored-activation (u kept by the forward) fwd+adj point")
     ap.add_argument("--no-refcfg", action="store_true",
@@ -165,6 +167,8 @@ def main():
 
     gx, gy = srk.kernel_spectra_srsrk(kw, sigma, M)
     E = patterns.grf_torch(B, N, seed=1000)
+    # f = I (differentiated) and g = E (convolved): the same images unless --f-ne-g
+    I = patterns.grf_torch(B, N, seed=2000) if args.f_ne_g else E
     gen = torch.Generator(device="cuda")
     gen.manual_seed(7)
     v = torch.randn((B, W, W), generator=gen, device="cuda")
@@ -179,12 +183,12 @@ def main():
 
     def fwd():
         if M == N:
-            srk.srkTrainingForward(E, E, gx, gy, N, W, e, dx, B, ws, R)
+            srk.srkTrainingForward(E, I, gx, gy, N, W, e, dx, B, ws, R)
         else:
-            srk.forward_batched(E, E, gx, gy, e, dx, ws, R)
+            srk.forward_batched(E, I, gx, gy, e, dx, ws, R)
 
     def adj():
-        srk.adjoint_batched(v, E, E, gx, gy, e, dx, ws, lf, lg)
+        srk.adjoint_batched(v, E, I, gx, gy, e, dx, ws, lf, lg)
 
     def step():
         fwd()
@@ -287,7 +291,7 @@ def main():
     graph = not args.no_graph
 
     value = B * N * N / (ms / 1e3) / 1e6
-    nbytes = alg_bytes(N, W, B, True)
+    nbytes = alg_bytes(N, W, B, not args.f_ne_g)
     achieved = nbytes / (ms / 1e3) / 1e9
     info = ws.info()
     traffic = pmc_traffic(N, B, kw, M, "fp32")
@@ -301,8 +305,8 @@ def main():
         uy = torch.empty((B, W, W), device="cuda")
 
         def step_u():
-            srk.forward_batched_u(E, E, gx, gy, e, dx, ws, R, ux, uy)
-            srk.adjoint_batched_u(v, ux, uy, E, gx, gy, e, dx, ws, lf, lg)
+            srk.forward_batched_u(E, I, gx, gy, e, dx, ws, R, ux, uy)
+            srk.adjoint_batched_u(v, ux, uy, I, gx, gy, e, dx, ws, lf, lg)
 
         for _ in range(args.warmup):
             step_u()
@@ -474,7 +478,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded Gaussian random fields, random cotangent)",
         "config": {"workload": f"srkTrainingForward + srkAdjointBatched (u recomputed), "
-                               f"N={N} B={B} KW={kw} erosion={e} W={W} E=I, FP32",
+                               f"N={N} B={B} KW={kw} erosion={e} W={W} "
+                               f"{'f != g' if args.f_ne_g else 'E=I'}, FP32",
                    "N": N, "B": B, "KW": kw, "erosion": e, "W": W, "M": M,
                    "tiles_per_chunk_fwd": info["chunk_fwd"], "tiles_per_chunk_adj": info["chunk_adj"],
                    "hipgraph": graph,
