@@ -106,3 +106,22 @@ def test_odd_geometry_tangent_pair(orc, N, kw, M):
     ra, rb = orc.tangent_adjoint(f, g, df, dg, lam, dlam, e, Kx, Ky)
     assert_close(host(a), ra, "dlam_f")
     assert_close(host(b), rb, "dlam_g")
+
+
+def test_largest_tile_fp32(orc):
+    """M = 2048 tiles (radix 16·16·8, one LDS line per band row, pad every 16
+    slots) in the FP32 form: forward and adjoint over a partial second tile."""
+    N, kw, M = 2300, 61, 2048
+    e = (kw - 1) // 2
+    W = N - 2 * e
+    gx, gy, Kx, Ky = kernels(kw, 10.0, M)
+    f = patterns.grf(N, 4242)[0]
+    g = patterns.grf(N, 4343)[0]
+    v = np.random.default_rng(2048).standard_normal((W, W)).astype(np.float32)
+    ws = srk.Workspace(N, W, e, M)
+    h = srk.forward_batched(dev(g[None]), dev(f[None]), gx, gy, e, 1.0, ws)
+    assert_close(host(h)[0], orc.forward(f, g, e, Kx, Ky, fft=True), "h")
+    lf, lg = srk.adjoint_batched(dev(v[None]), dev(g[None]), dev(f[None]), gx, gy, e, 1.0, ws)
+    rf, rg = orc.adjoint(v, f, g, e, Kx, Ky, fft=True)
+    assert_close(host(lf)[0], rf, "lambda_f")
+    assert_close(host(lg)[0], rg, "lambda_g")
