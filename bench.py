@@ -70,6 +70,14 @@ def pass_bytes_per_tile(N, M, e):
     }
 
 
+def ref_train_ws_bytes(N, B):
+    """Device bytes of the reference's batched forward workspace
+    (SrkTrainWorkspace::allocate, training/srk_training_forward.cuh:130-141): five
+    real [B x N x N] arrays and two complex [B x N x (N/2+1)] arrays; cuFFT work
+    areas are not counted."""
+    return 5 * 4 * B * N * N + 2 * 8 * B * N * (N // 2 + 1)
+
+
 def cpu_cores():
     try:
         return len(os.sched_getaffinity(0))
@@ -176,6 +184,15 @@ def main():
     lf = torch.empty((B, N, N), device="cuda")
     lg = torch.empty((B, N, N), device="cuda")
     ws = srk.Workspace(N, W, e, M, B_max=B, scratch_mb=args.scratch_mb)
+    step_tensors = [E, v, R, lf, lg] + ([I] if args.f_ne_g else [])
+    memory = {
+        "workspace_bytes": ws.nbytes,
+        "caller_buffers_bytes": sum(t.numel() * t.element_size() for t in step_tensors),
+        "reference_training_workspace_bytes": ref_train_ws_bytes(N, B),
+        "note": "device memory the library holds for the whole fwd+adj step (exchange "
+                "buffers, spectra, tile tables) against the reference's forward-only batched "
+                "workspace at the same N and B; the inputs and outputs are the caller's",
+    }
     ws.set_tuning(chunk_tiles=args.chunk, p2_rows=args.p2_rows,
                   concurrency=None if args.concurrency < 0 else args.concurrency)
     stream = torch.cuda.current_stream()
@@ -486,6 +503,7 @@ def main():
                    "precision": args.precision if args.precision in reduced else "fp32"},
         "ms_fwd": round(t_f, 3),
         "ms_adj": round(t_a, 3),
+        "memory": memory,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,

@@ -44,3 +44,10 @@ def test_pmc_traffic_summaries_parse():
     # measured traffic lies between the algorithmic bytes and the exchange-heavy bound
     alg = bench.alg_bytes(512, 392, 5000, True)
     assert alg < bf16 < fp32 < 10 * alg
+
+
+def test_reference_training_workspace_size():
+    # 36.74 GB at N = 512, B = 5000; 5.52 GB at N = 256, B = 3000 (the reference's notebook
+    # reports 6.63 GB in use there, cuFFT work areas included)
+    assert bench.ref_train_ws_bytes(512, 5000) == 36741120000
+    assert abs(bench.ref_train_ws_bytes(256, 3000) / 1e9 - 5.52) < 0.01
