@@ -537,10 +537,14 @@ void k_p2_conv(EngineArgs a, int nf, int tiles)
                 const float2* gxr = a.GT + (long long)(k0 + w) * M + lane;
                 const float2* gyr = a.GT + (long long)(H + k0 + w) * M + lane;
                 float2 X[V];
+                // the Gy loads are in flight during the forward transform (the
+                // Gx row as well would spill under the 128-register bound)
+#pragma unroll
+                for (int i = 0; i < V; ++i) U[i] = ldu(gyr, kWave * i);
                 fft_lds_to_regs<M, -1>(FP, la, X, lane);
 #pragma unroll
                 for (int i = 0; i < V; ++i) {
-                    U[i] = cmul(X[i], ldu(gyr, kWave * i));
+                    U[i] = cmul(X[i], U[i]);
                     X[i] = cmul(X[i], ldu(gxr, kWave * i));
                 }
                 fft_from_regs<M, +1>(FP, la, X, lane);
@@ -643,15 +647,26 @@ void k_p2_corr(EngineArgs a, int planeIn, int planeOut, int tiles)
             if (row) fft_lds_to_regs<M, -1>(FP, la, X, lane);
             __syncthreads();  // every line is read: the lines take the second plane
             pb.put(A);
+            // the conj(Gx) loads are in flight across the barrier, the conj(Gy)
+            // loads during the second transform
+            const float2* gxr = a.GT + (long long)(k0 + w) * M + lane;
+            const float2* gyr = a.GT + (long long)(H + k0 + w) * M + lane;
+            float2 G[V];
+            if (row) {
+#pragma unroll
+                for (int i = 0; i < V; ++i) G[i] = ldu(gxr, kWave * i);
+            }
             __syncthreads();
             if (row) {
-                const float2* gxr = a.GT + (long long)(k0 + w) * M + lane;
-                const float2* gyr = a.GT + (long long)(H + k0 + w) * M + lane;
+#pragma unroll
+                for (int i = 0; i < V; ++i) {
+                    X[i] = cmulc(X[i], G[i]);
+                    G[i] = ldu(gyr, kWave * i);
+                }
                 float2 Y[V];
                 fft_lds_to_regs<M, -1>(FP, la, Y, lane);
 #pragma unroll
-                for (int i = 0; i < V; ++i)
-                    X[i] = cadd(cmulc(X[i], ldu(gxr, kWave * i)), cmulc(Y[i], ldu(gyr, kWave * i)));
+                for (int i = 0; i < V; ++i) X[i] = cadd(X[i], cmulc(Y[i], G[i]));
                 fft_from_regs<M, +1>(FP, la, X, lane);
             }
             __syncthreads();
