@@ -140,6 +140,8 @@ def main():
                     help="main point with distinct images for f (I) and g (E)")
     ap.add_argument("--no-stored-u", action="store_true",
                     help="skip the stored-activation (u kept by the forward) fwd+adj point")
+    ap.add_argument("--no-cd", action="store_true",
+                    help="skip the batched CD measurement on the main point's output")
     ap.add_argument("--no-refcfg", action="store_true",
                     help="skip the forward at the configuration of the reference's published timing")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
@@ -354,6 +356,28 @@ def main():
                   "stored_activation_bytes": 8 * B * W * W}
         del gu, ux, uy
 
+    cd_out = None
+    if not args.no_cd:
+        # the training CD sweep's measurement (training/main_cd_training.cu:213-261):
+        # crop, E + c R, centre row, polyphase upsampling, edges and CD of every image,
+        # fused into one launch over the batch
+        def cd_call():
+            return srk.shrinkage_cd_batched(E, R, e, 1.0, center_row=True, threshold=0.5)
+
+        cd_call()
+        torch.cuda.synchronize()
+        reps = 20
+        start.record(stream)
+        for _ in range(reps):
+            cd_call()
+        stop.record(stream)
+        torch.cuda.synchronize()
+        ms_cd = start.elapsed_time(stop) / reps
+        cd_out = {"workload": f"srkShrinkageCDBatched: centre-row CD of E_crop + c R, B={B}, "
+                              f"W={W}, L=16, A=8, threshold 0.5",
+                  "ms_per_call": round(ms_cd, 4),
+                  "signals_per_s": round(B / (ms_cd / 1e3), 1)}
+
     secondary = None
     if not args.no_large:
         del lf, lg, R, v, E, ws
@@ -528,6 +552,7 @@ def main():
         "secondary": secondary,
         "ragged": ragged_out,
         "stored_u": stored,
+        "cd_metric": cd_out,
         "reference_config_forward": refcfg,
         "cpu_baseline": cpu,
     }
