@@ -149,9 +149,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: the shrinkage operator has no multi-GPU path "
-              "in this build (batch sharding across GPUs is a design note, not code). "
-              "Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU only: the shrinkage operator library has no "
+              "multi-GPU path (sharding the batch across GPUs is described, not "
+              "implemented). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import torch
