@@ -78,6 +78,25 @@ def ref_train_ws_bytes(N, B):
     return 5 * 4 * B * N * N + 2 * 8 * B * N * (N // 2 + 1)
 
 
+def device_copy_GBps(torch, mb=1024, reps=10):
+    """Read + write bandwidth of a plain device-to-device copy of `mb` MiB (the
+    rate a bandwidth-bound pass can reach on this device)."""
+    n = mb * (1 << 20) // 4
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    s, t = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        b.copy_(a)
+    t.record()
+    torch.cuda.synchronize()
+    ms = s.elapsed_time(t) / reps
+    del a, b
+    return 2 * n * 4 / (ms / 1e3) / 1e9
+
+
 def cpu_cores():
     try:
         return len(os.sched_getaffinity(0))
@@ -314,6 +333,8 @@ def main():
     achieved = nbytes / (ms / 1e3) / 1e9
     info = ws.info()
     traffic = pmc_traffic(N, B, kw, M, "fp32")
+    copy_gbs = device_copy_GBps(torch)
+    traffic_gbs = traffic / (ms / 1e3) / 1e9 if traffic else None
 
     stored = None
     if not args.no_stored_u:
@@ -531,10 +552,16 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
+                     "traffic_GBps": round(traffic_gbs, 1) if traffic_gbs else None,
+                     "traffic_frac": round(traffic_gbs / HBM_PEAK_GBS, 4) if traffic_gbs else None,
+                     "copy_GBps": round(copy_gbs, 1),
+                     "traffic_frac_of_copy": round(traffic_gbs / copy_gbs, 4) if traffic_gbs else None,
                      "scope": "one fwd+adj step over the batch (one hipGraph launch of all "
                               "passes); algorithmic bytes 16N^2+8W^2 per image = "
                               f"{nbytes / 1e9:.2f} GB per step; traffic = HBM bytes per step "
-                              "from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary in profiles/",
+                              "from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary in profiles/, "
+                              "traffic_GBps = traffic / step time; copy_GBps = a plain 1 GiB "
+                              "device copy (read + write) timed in the same run",
                      "dominant_kernel": dom,
                      "dominant_kernel_pass": (
                          {"kernel": dom, "bytes_per_launch": kernels[dom]["pass_bytes_per_launch"],
