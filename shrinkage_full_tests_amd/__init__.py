@@ -39,6 +39,31 @@ def _ptr(t) -> int:
     return 0 if t is None else int(t.data_ptr())
 
 
+def _dev(t, numel: int, what: str, dtype: str = "float32", optional: bool = False) -> int:
+    """Device address of a tensor handed to the library: a contiguous CUDA
+    tensor of `dtype` with at least `numel` elements.  Anything else raises
+    ValueError here instead of reaching a kernel as a wrong or host address."""
+    torch = _torch()
+    if t is None:
+        if optional:
+            return 0
+        raise ValueError(f"{what}: a tensor is required")
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise ValueError(f"{what}: needs a CUDA tensor")
+    if t.dtype != getattr(torch, dtype):
+        raise ValueError(f"{what}: dtype {t.dtype}, expected torch.{dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{what}: needs a contiguous tensor")
+    if t.numel() < numel:
+        raise ValueError(f"{what}: {t.numel()} elements, needs {numel}")
+    return int(t.data_ptr())
+
+
+def _spec(G, ws, what: str) -> int:
+    """Reference-layout spectrum [M, M/2+1] complex64 at the workspace's tile size."""
+    return _dev(G, ws.M * (ws.M // 2 + 1), what, "complex64")
+
+
 def _stream(stream) -> int:
     torch = _torch()
     if stream is None:
@@ -176,35 +201,46 @@ def precompute_kernel_freq(G, M: int, device="cuda"):
 # ---- drop-in entry points (reference names) --------------------------------
 
 def srkTrainingForward(E, I, Gx_freq, Gy_freq, N, W, erosion, dx, B, ws, R, stream=None):
-    call("srkTrainingForward", _ptr(E), _ptr(I), _ptr(Gx_freq), _ptr(Gy_freq), N, W, erosion,
-         float(dx), B, _stream(stream), _ptr(R), ws.handle)
+    img, val = B * N * N, B * W * W
+    call("srkTrainingForward", _dev(E, img, "E"), _dev(I, img, "I"), _spec(Gx_freq, ws, "Gx"),
+         _spec(Gy_freq, ws, "Gy"), N, W, erosion, float(dx), B, _stream(stream),
+         _dev(R, val, "R"), ws.handle)
     return R
 
 
 def srkForwardOaS(E, I, Gx_freq, Gy_freq, N, W, erosion, dx, ws, R, ux=None, uy=None,
                   stream=None):
-    call("srkForwardOaS", _ptr(E), _ptr(I), _ptr(Gx_freq), _ptr(Gy_freq), N, W, erosion,
-         float(dx), _stream(stream), _ptr(R), ws.handle, _ptr(ux), _ptr(uy))
+    img, val = N * N, W * W
+    call("srkForwardOaS", _dev(E, img, "E"), _dev(I, img, "I"), _spec(Gx_freq, ws, "Gx"),
+         _spec(Gy_freq, ws, "Gy"), N, W, erosion, float(dx), _stream(stream), _dev(R, val, "R"),
+         ws.handle, _dev(ux, val, "ux", optional=True), _dev(uy, val, "uy", optional=True))
     return R
 
 
 def adjointSRK(v, ux, uy, I, Gx_freq, Gy_freq, N, W, erosion, dx, ws, dLdI, dLdE, stream=None):
-    call("adjointSRK", _ptr(v), _ptr(ux), _ptr(uy), _ptr(I), _ptr(Gx_freq), _ptr(Gy_freq), N, W,
-         erosion, float(dx), _stream(stream), _ptr(dLdI), _ptr(dLdE), ws.handle)
+    img, val = N * N, W * W
+    call("adjointSRK", _dev(v, val, "v"), _dev(ux, val, "ux"), _dev(uy, val, "uy"),
+         _dev(I, img, "I"), _spec(Gx_freq, ws, "Gx"), _spec(Gy_freq, ws, "Gy"), N, W, erosion,
+         float(dx), _stream(stream), _dev(dLdI, img, "dLdI"), _dev(dLdE, img, "dLdE"),
+         ws.handle)
     return dLdI, dLdE
 
 
 def shrinkageTangentOaS(f, g, df, dg, Gx_freq, Gy_freq, N, W, erosion, dx, ws, dh, stream=None):
-    call("shrinkageTangentOaS", _ptr(f), _ptr(g), _ptr(df), _ptr(dg), _ptr(Gx_freq),
-         _ptr(Gy_freq), N, W, erosion, float(dx), _stream(stream), _ptr(dh), ws.handle)
+    img, val = N * N, W * W
+    call("shrinkageTangentOaS", _dev(f, img, "f"), _dev(g, img, "g"), _dev(df, img, "df"),
+         _dev(dg, img, "dg"), _spec(Gx_freq, ws, "Gx"), _spec(Gy_freq, ws, "Gy"), N, W, erosion,
+         float(dx), _stream(stream), _dev(dh, val, "dh"), ws.handle)
     return dh
 
 
 def shrinkageTangentAdjointOaS(f, g, df, dg, lam_h, dlam_h, Gx_freq, Gy_freq, N, W, erosion, dx,
                                ws, dLam_f, dLam_g, stream=None):
-    call("shrinkageTangentAdjointOaS", _ptr(f), _ptr(g), _ptr(df), _ptr(dg), _ptr(lam_h),
-         _ptr(dlam_h), _ptr(Gx_freq), _ptr(Gy_freq), N, W, erosion, float(dx), _stream(stream),
-         _ptr(dLam_f), _ptr(dLam_g), ws.handle)
+    img, val = N * N, W * W
+    call("shrinkageTangentAdjointOaS", _dev(f, img, "f"), _dev(g, img, "g"), _dev(df, img, "df"),
+         _dev(dg, img, "dg"), _dev(lam_h, val, "lam_h"), _dev(dlam_h, val, "dlam_h"),
+         _spec(Gx_freq, ws, "Gx"), _spec(Gy_freq, ws, "Gy"), N, W, erosion, float(dx),
+         _stream(stream), _dev(dLam_f, img, "dLam_f"), _dev(dLam_g, img, "dLam_g"), ws.handle)
     return dLam_f, dLam_g
 
 
@@ -216,8 +252,10 @@ def forward_batched(E, I, Gx_freq, Gy_freq, erosion, dx, ws, R=None, stream=None
     W = N - 2 * erosion
     if R is None:
         R = torch.empty((B, W, W), dtype=torch.float32, device=E.device)
-    call("srkForwardBatched", _ptr(E), _ptr(I), _ptr(Gx_freq), _ptr(Gy_freq), N, W, erosion,
-         float(dx), B, _stream(stream), _ptr(R), ws.handle)
+    img, val = B * N * N, B * W * W
+    call("srkForwardBatched", _dev(E, img, "E"), _dev(I, img, "I"), _spec(Gx_freq, ws, "Gx"),
+         _spec(Gy_freq, ws, "Gy"), N, W, erosion, float(dx), B, _stream(stream),
+         _dev(R, val, "R"), ws.handle)
     return R
 
 
@@ -230,8 +268,10 @@ def adjoint_batched(v, E, I, Gx_freq, Gy_freq, erosion, dx, ws, dLdI=None, dLdE=
         dLdI = torch.empty((B, N, N), dtype=torch.float32, device=E.device)
     if dLdE is None:
         dLdE = torch.empty((B, N, N), dtype=torch.float32, device=E.device)
-    call("srkAdjointBatched", _ptr(v), _ptr(E), _ptr(I), _ptr(Gx_freq), _ptr(Gy_freq), N, W,
-         erosion, float(dx), B, _stream(stream), _ptr(dLdI), _ptr(dLdE), ws.handle)
+    img, val = B * N * N, B * W * W
+    call("srkAdjointBatched", _dev(v, val, "v"), _dev(E, img, "E"), _dev(I, img, "I"),
+         _spec(Gx_freq, ws, "Gx"), _spec(Gy_freq, ws, "Gy"), N, W, erosion, float(dx), B,
+         _stream(stream), _dev(dLdI, img, "dLdI"), _dev(dLdE, img, "dLdE"), ws.handle)
     return dLdI, dLdE
 
 
@@ -243,8 +283,10 @@ def forward_batched_u(E, I, Gx_freq, Gy_freq, erosion, dx, ws, R=None, ux=None, 
     W = N - 2 * erosion
     R, ux, uy = (t if t is not None else torch.empty((B, W, W), dtype=torch.float32,
                                                        device=E.device) for t in (R, ux, uy))
-    call("srkForwardBatchedU", _ptr(E), _ptr(I), _ptr(Gx_freq), _ptr(Gy_freq), N, W, erosion,
-         float(dx), B, _stream(stream), _ptr(R), _ptr(ux), _ptr(uy), ws.handle)
+    img, val = B * N * N, B * W * W
+    call("srkForwardBatchedU", _dev(E, img, "E"), _dev(I, img, "I"), _spec(Gx_freq, ws, "Gx"),
+         _spec(Gy_freq, ws, "Gy"), N, W, erosion, float(dx), B, _stream(stream),
+         _dev(R, val, "R"), _dev(ux, val, "ux"), _dev(uy, val, "uy"), ws.handle)
     return R, ux, uy
 
 
@@ -258,8 +300,10 @@ def adjoint_batched_u(v, ux, uy, I, Gx_freq, Gy_freq, erosion, dx, ws, dLdI=None
         dLdI = torch.empty((B, N, N), dtype=torch.float32, device=I.device)
     if dLdE is None:
         dLdE = torch.empty((B, N, N), dtype=torch.float32, device=I.device)
-    call("srkAdjointBatchedU", _ptr(v), _ptr(ux), _ptr(uy), _ptr(I), _ptr(Gx_freq),
-         _ptr(Gy_freq), N, W, erosion, float(dx), B, _stream(stream), _ptr(dLdI), _ptr(dLdE),
+    img, val = B * N * N, B * W * W
+    call("srkAdjointBatchedU", _dev(v, val, "v"), _dev(ux, val, "ux"), _dev(uy, val, "uy"),
+         _dev(I, img, "I"), _spec(Gx_freq, ws, "Gx"), _spec(Gy_freq, ws, "Gy"), N, W, erosion,
+         float(dx), B, _stream(stream), _dev(dLdI, img, "dLdI"), _dev(dLdE, img, "dLdE"),
          ws.handle)
     return dLdI, dLdE
 
@@ -270,8 +314,10 @@ def tangent_batched(f, g, df, dg, Gx_freq, Gy_freq, erosion, dx, ws, dh=None, st
     W = N - 2 * erosion
     if dh is None:
         dh = torch.empty((B, W, W), dtype=torch.float32, device=f.device)
-    call("srkTangentBatched", _ptr(f), _ptr(g), _ptr(df), _ptr(dg), _ptr(Gx_freq), _ptr(Gy_freq),
-         N, W, erosion, float(dx), B, _stream(stream), _ptr(dh), ws.handle)
+    img, val = B * N * N, B * W * W
+    call("srkTangentBatched", _dev(f, img, "f"), _dev(g, img, "g"), _dev(df, img, "df"),
+         _dev(dg, img, "dg"), _spec(Gx_freq, ws, "Gx"), _spec(Gy_freq, ws, "Gy"), N, W, erosion,
+         float(dx), B, _stream(stream), _dev(dh, val, "dh"), ws.handle)
     return dh
 
 
@@ -284,9 +330,11 @@ def tangent_adjoint_batched(f, g, df, dg, lam_h, dlam_h, Gx_freq, Gy_freq, erosi
         dLam_f = torch.empty((B, N, N), dtype=torch.float32, device=f.device)
     if dLam_g is None:
         dLam_g = torch.empty((B, N, N), dtype=torch.float32, device=f.device)
-    call("srkTangentAdjointBatched", _ptr(f), _ptr(g), _ptr(df), _ptr(dg), _ptr(lam_h),
-         _ptr(dlam_h), _ptr(Gx_freq), _ptr(Gy_freq), N, W, erosion, float(dx), B,
-         _stream(stream), _ptr(dLam_f), _ptr(dLam_g), ws.handle)
+    img, val = B * N * N, B * W * W
+    call("srkTangentAdjointBatched", _dev(f, img, "f"), _dev(g, img, "g"), _dev(df, img, "df"),
+         _dev(dg, img, "dg"), _dev(lam_h, val, "lam_h"), _dev(dlam_h, val, "dlam_h"),
+         _spec(Gx_freq, ws, "Gx"), _spec(Gy_freq, ws, "Gy"), N, W, erosion, float(dx), B,
+         _stream(stream), _dev(dLam_f, img, "dLam_f"), _dev(dLam_g, img, "dLam_g"), ws.handle)
     return dLam_f, dLam_g
 
 
@@ -299,8 +347,8 @@ def measure_cd_batched(sig, L: int = 16, A: int = 8, threshold: float = 0.0, str
     sig = sig.contiguous()
     B, N = sig.shape
     out = [torch.empty(B, dtype=torch.float32, device=sig.device) for _ in range(3)]
-    call("srkMeasureCDBatched", _ptr(sig), B, N, L, A, float(threshold), *map(_ptr, out),
-         _stream(stream))
+    call("srkMeasureCDBatched", _dev(sig, B * N, "signals"), B, N, L, A, float(threshold),
+         *map(_ptr, out), _stream(stream))
     return tuple(out)
 
 
@@ -312,7 +360,8 @@ def shrinkage_cd_batched(E, R, erosion: int, coeff_c: float = 1.0, center_row: b
     B, N = E.shape[0], E.shape[-1]
     W = R.shape[-1]
     out = [torch.empty(B, dtype=torch.float32, device=E.device) for _ in range(3)]
-    call("srkShrinkageCDBatched", _ptr(E), _ptr(R), N, W, erosion, B, float(coeff_c),
+    call("srkShrinkageCDBatched", _dev(E, B * N * N, "E"), _dev(R, B * W * W, "R"), N, W,
+         erosion, B, float(coeff_c),
          int(bool(center_row)), L, A, float(threshold), *map(_ptr, out), _stream(stream))
     return tuple(out)
 

@@ -111,3 +111,19 @@ def test_cd_entry_points_reject_bad_arguments():
     assert L.srkMeasureCDBatched(p, 1, 1, 16, 8, 0.0, p, p, p, None) == -2
     assert L.srkMeasureCDBatched(p, 1, 50000, 16, 8, 0.0, p, p, p, None) == -5
     assert L.srkShrinkageCDBatched(p, p, 256, 100, 60, 1, 1.0, 1, 16, 8, 0.0, p, p, p, None) == -2
+
+
+def test_host_tensors_rejected_before_the_library():
+    import torch
+
+    import shrinkage_full_tests_amd as srk
+
+    class _Ws:           # stands in for a workspace; no device is touched
+        M, handle = 256, None
+
+    E = torch.zeros((1, 256, 256))
+    with pytest.raises(ValueError, match="CUDA"):
+        srk.forward_batched(E, E, None, None, 15, 1.0, _Ws())
+    with pytest.raises(ValueError, match="CUDA"):
+        srk.shrinkageTangentOaS(E[0], E[0], E[0], E[0], None, None, 256, 226, 15, 1.0, _Ws(),
+                                torch.zeros((226, 226)))
