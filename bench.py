@@ -104,12 +104,36 @@ def cpu_cores():
         return os.cpu_count() or 1
 
 
+def cpu_model():
+    """Model name of the host CPU from /proc/cpuinfo (platform fallback)."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.lower().startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def cpu_share():
+    """Host threads this process may use: the CPU share the machine grants one
+    GPU job (OMP_NUM_THREADS where it is set), else every core of the affinity
+    mask."""
+    aff = cpu_cores()
+    try:
+        share = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        share = 0
+    return min(aff, share) if share > 0 else aff
+
+
 def cpu_baseline(N, kw, sigma, e, budget_s=12.0):
     """Timed FP64 FFT-form forward + adjoint of the oracle on the host cores."""
     import numpy as np
     import oracle
     from shrinkage_full_tests_amd import patterns
-    threads = min(16, cpu_cores())
+    threads = cpu_share()
     oracle.set_threads(threads)
     Kx, Ky = oracle.srsrk_kernel(kw, 1.0, sigma)
     W = N - 2 * e
@@ -126,8 +150,14 @@ def cpu_baseline(N, kw, sigma, e, budget_s=12.0):
     dt = time.perf_counter() - t0
     return {"value": round(n * N * N / dt / 1e6, 3), "unit": "Mpix/s", "cores": threads,
             "kind": "port",
+            "cpu_model": cpu_model(),
+            "cpu_count": os.cpu_count(),
+            "affinity_cores": cpu_cores(),
+            "threads_used": threads,
             "sample": f"{n} image(s) N={N} KW={kw} E=I, FP64 FFT-form fwd+adj (oracle), "
-                      f"{dt:.1f} s on {platform.processor() or platform.machine()}"}
+                      f"{dt:.1f} s on {threads} threads (the CPU share of one GPU job; "
+                      f"{cpu_cores()} cores in the affinity mask, os.cpu_count() = "
+                      f"{os.cpu_count()}) of {cpu_model()}"}
 
 
 def main():

@@ -27,6 +27,13 @@
  *     the odd SRSRK kernels.
  *   - The workspace may be shared by all entry points and batch sizes up to
  *     B_max (the reference asserts B == B_max, training/srk_training_forward.cuh:186).
+ *   - Calls on one workspace share its buffers.  A call on a stream other than
+ *     the previous call's waits for that call (a per-workspace completion
+ *     event), so one host thread may use one workspace from several streams.
+ *     Calls being captured into a graph are not ordered this way: replaying a
+ *     graph while calling on the same workspace from another stream races.
+ *     Calls from several host threads on one workspace need the caller's
+ *     locking (the reference's workspaces are single-threaded too).
  */
 #ifndef SRK_AMD_H
 #define SRK_AMD_H
@@ -109,6 +116,21 @@ int srkKernelSpectraSRSRKDevice(int kw, double sigma, double dx, double cw, int 
                                 void* d_Gx_freq, void* d_Gy_freq, float* d_Kx, float* d_Ky,
                                 hipStream_t stream);
 
+/*
+ * Support check of a spectrum pair of size Ns (reference layout, device):
+ * the relative RMS of the Ns-periodic spatial kernel outside the erosion box
+ * |x|, |y| <= erosion, evaluated in FP64 on the device (inverse DFT on the
+ * box, Parseval for the whole kernel), is written to *ratio (optional);
+ * returns SRK_ERR_KERNEL if it exceeds 1e-4, i.e. if the erosion does not
+ * cover the kernel radius (erosion < (KW-1)/2) and the valid region would see
+ * wrapped or truncated taps.  erosion must be the workspace's; Ns <= max(N, M)
+ * of the workspace.  Synchronous (waits for the stream).  No counterpart in
+ * the reference, whose drivers choose erosion >= (KW-1)/2 themselves
+ * (training/main_cd_training.cu:302, adjoint/main_adjoint.cu:254).
+ */
+int srkCheckSpectraSupport(const void* Gx_freq, const void* Gy_freq, int Ns, int erosion,
+                           hipStream_t stream, srk_workspace_t ws, double* ratio);
+
 /* ---- workspace ---------------------------------------------------------- */
 
 /*
@@ -162,8 +184,17 @@ int srkSynchronize(hipStream_t stream);
 /*
  * Replaces srkTrainingForward (training/srk_training_forward.cuh:175-241):
  * R_b = dx^2 (conv(E_b,Gx) dI_b/dx + conv(E_b,Gy) dI_b/dy) on the valid region.
- * E, I: [B x N x N]; R: [B x W x W]; spectra at size N.  E may equal I.
- * This is also the op_test() contract of training/main_cd_training.cu:101-121.
+ * E, I: [B x N x N]; R: [B x W x W]; spectra at size N (any N, as the
+ * reference plans its transforms at any N, training/srk_training_forward.cuh:
+ * 130-158).  E may equal I.  This is also the op_test() contract of
+ * training/main_cd_training.cu:101-121.
+ * With a workspace of tile size M == N the call is asynchronous.  With
+ * M != N (e.g. N = 192 or 288 on M = 256 or 512 tiles) the spatial kernel is
+ * recovered on the erosion box |x|, |y| <= erosion from the N-size spectra
+ * (FP64, on the device) and transformed again at size M; the call then waits
+ * for the support measure of srkCheckSpectraSupport and returns
+ * SRK_ERR_KERNEL if the kernel reaches beyond the box (not checked while the
+ * stream is being captured).
  */
 int srkTrainingForward(const float* E, const float* I, const void* Gx_freq, const void* Gy_freq,
                        int N, int W, int erosion, float dx, int B, hipStream_t stream, float* R,
@@ -276,14 +307,47 @@ int srkGpGenerate(double sigma, double ell_x, double ell_y, double rho, int W, i
  * with a 2A+1-tap sinc x Blackman polyphase filter, find the nearest crossings
  * left of and at-or-right of the centre (N L - 1)/2 and write, per signal,
  * left and right edge in pixels and cd = right - left (device arrays [B]; a
- * missing edge is -inf / +inf).  The reference returns a host vector and
- * synchronises; this call is asynchronous on the stream and allocates
- * nothing.  threshold = 0 measures zero crossings as the reference does.
+ * missing edge reads -FLT_MAX / L (left) or +FLT_MAX / L (right), finite, as
+ * the reference's search start values, cd_measure_util.cu:37,218-219,264-282).
+ * The reference returns a host vector and synchronises; this call is
+ * asynchronous on the stream and allocates nothing.  threshold = 0 measures
+ * zero crossings as the reference does.
  * Limits: N * L <= 2^24 and N + L(2A+1) + 512 floats within 160 KB of LDS.
  */
 int srkMeasureCDBatched(const float* d_in, int B, int N, int L, int A, float threshold,
                         float* d_left_px, float* d_right_px, float* d_cd_px,
                         hipStream_t stream);
+
+/*
+ * Replaces BuildPolyphaseTable (cd_measure_util.cuh:11-18, cd_measure_util.cu:
+ * 285-296): the sinc x Blackman polyphase table [L x (2A+1)], row-major, rows
+ * normalised to sum 1, written to the caller's device buffer d_coeff (the
+ * reference allocates it with cudaMallocAsync; this library never allocates
+ * in a call).  Asynchronous.
+ */
+int srkBuildPolyphaseTable(int L, int A, float* d_coeff, hipStream_t stream);
+
+/*
+ * Replaces UpsampleBatched (cd_measure_util.cuh:20-33, cd_measure_util.cu:
+ * 298-308): d_out[b][k L + r] = sum_i d_in[b][k + i - A] d_coeff[r][i], taps
+ * outside the signal read zero; d_in [B x N], d_out [B x N L] row-major,
+ * caller-allocated.  Asynchronous.
+ */
+int srkUpsampleBatched(const float* d_in, const float* d_coeff, int B, int N, int L, int A,
+                       float* d_out, hipStream_t stream);
+
+/*
+ * Replaces the MeasureCD_Batched overload taking a prebuilt table and / or
+ * upsampled signals (cd_measure_util.cuh:80-95, cd_measure_util.cu:310-332):
+ * zero-crossing CD of B signals of length N.  d_coeff (nullable): the table
+ * of srkBuildPolyphaseTable, else built internally; d_up (nullable): signals
+ * already upsampled by srkUpsampleBatched [B x N L], else d_in [B x N] is
+ * upsampled on the fly.  Outputs and missing edges as srkMeasureCDBatched.
+ * Asynchronous, allocates nothing.
+ */
+int srkMeasureCDBatchedEx(const float* d_in, const float* d_coeff, const float* d_up, int B, int N,
+                          int L, int A, float* d_left_px, float* d_right_px, float* d_cd_px,
+                          hipStream_t stream);
 
 /*
  * CD of the shrinkage result, fused: signal = E_crop + coeff_c * R on the centre

@@ -180,3 +180,26 @@ def adjoint_window(v, f, g, e, Kx, Ky, qx0, qy0, wx, wy, s=1.0):
     lib().orc_adjoint_window(_p(v), _p(f), _p(g), f.shape[0], int(e), _p(Kx), _p(Ky), Kx.shape[0],
                              float(s), int(qx0), int(qy0), int(wx), int(wy), _p(a), _p(b))
     return a, b
+
+
+def tangent_window(f, g, df, dg, e, Kx, Ky, ox0, oy0, wx, wy, s=1.0):
+    """Tangent dh on an output window (valid-region coordinates).
+
+    The forward is bilinear in (f, g), so its tangent
+    dh = u_x(dg) f_x + u_y(dg) f_y + u_x(g) df_x + u_y(g) df_y
+    (tan_tanadj/srk_tangent.cuh:28-52) is h(f, dg) + h(df, g), evaluated here
+    with two direct forward windows."""
+    return (forward_window(f, dg, e, Kx, Ky, ox0, oy0, wx, wy, s) +
+            forward_window(df, g, e, Kx, Ky, ox0, oy0, wx, wy, s))
+
+
+def tangent_adjoint_window(f, g, df, dg, lam_h, dlam_h, e, Kx, Ky, qx0, qy0, wx, wy, s=1.0):
+    """(dLam_f, dLam_g) on an image window.
+
+    dLam_f = A_f(dlam_h; u(g)) + A_f(lam_h; u(dg)) and
+    dLam_g = A_g(dlam_h; f) + A_g(lam_h; df)
+    (tan_tanadj/srk_tangent_adjoint.cuh:100-159), i.e. the adjoint window of
+    (dlam_h, f, g) plus the adjoint window of (lam_h, df, dg)."""
+    a1, b1 = adjoint_window(dlam_h, f, g, e, Kx, Ky, qx0, qy0, wx, wy, s)
+    a2, b2 = adjoint_window(lam_h, df, dg, e, Kx, Ky, qx0, qy0, wx, wy, s)
+    return a1 + a2, b1 + b2

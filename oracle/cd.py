@@ -8,8 +8,11 @@ Follows third_party/src/metricUtil/cd_measure_util.cu:
     x = i - y0/(y1 - y0) (:134-170);
   - nearest crossing left of / at-or-right of the centre (N*L - 1)/2 (:205-256,
     centre at :421);
-  - CD = (right - left)/L pixels (:264-282).
-Computed in double precision.
+  - CD = (right - left)/L pixels (:264-282);
+  - a missing edge keeps the search's start value -/+FLT_MAX (:37, 218-219), so
+    it reads -/+FLT_MAX/L pixels, and cd = right - left in FP32 (:264-282).
+Found edges are computed in double precision; the missing-edge values follow
+the reference's FP32 arithmetic exactly.
 """
 from __future__ import annotations
 
@@ -48,7 +51,7 @@ def upsample(sig: np.ndarray, L: int = 16, A: int = 8) -> np.ndarray:
 
 def measure_cd(sig: np.ndarray, L: int = 16, A: int = 8, threshold: float = 0.0):
     """CD in pixels of each row of sig [B, N] at the given threshold.
-    Returns (left_px, right_px, cd_px); missing crossings give -inf / +inf."""
+    Returns (left_px, right_px, cd_px); a missing crossing gives -/+FLT_MAX/L."""
     up = upsample(np.asarray(sig, np.float64) - threshold, L, A)
     B, Nup = up.shape
     y0, y1 = up[:, :-1], up[:, 1:]
@@ -59,4 +62,18 @@ def measure_cd(sig: np.ndarray, L: int = 16, A: int = 8, threshold: float = 0.0)
     centre = 0.5 * (Nup - 1)
     left = np.where(cross & (x < centre), x, -np.inf).max(axis=1)
     right = np.where(cross & (x >= centre), x, np.inf).min(axis=1)
-    return left / L, right / L, (right - left) / L
+    fl, fr = np.isfinite(left), np.isfinite(right)
+    inv = np.float32(1.0) / np.float32(L)
+    lpx = np.where(fl, left / L, np.float64(np.float32(-FLT_MAX) * inv))
+    rpx = np.where(fr, right / L, np.float64(np.float32(FLT_MAX) * inv))
+    cd32 = (rpx.astype(np.float32) - lpx.astype(np.float32)).astype(np.float64)
+    cd = np.where(fl & fr, (right - left) / L, cd32)
+    return lpx, rpx, cd
+
+
+FLT_MAX = float(np.finfo(np.float32).max)
+
+
+def found(px, L: int = 16) -> np.ndarray:
+    """True where an edge position (pixels) was found rather than missing."""
+    return np.abs(np.asarray(px, np.float64)) < 0.5 * FLT_MAX / L

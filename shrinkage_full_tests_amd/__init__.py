@@ -21,7 +21,8 @@ __all__ = [
     "shrinkageTangentAdjointOaS", "forward_batched", "adjoint_batched", "tangent_batched",
     "forward_batched_u", "adjoint_batched_u",
     "tangent_adjoint_batched", "ShrinkageFunction", "erosion_for", "SUPPORTED_M",
-    "measure_cd_batched", "shrinkage_cd_batched",
+    "measure_cd_batched", "shrinkage_cd_batched", "check_spectra_support",
+    "build_polyphase_table", "upsample_batched", "measure_cd_batched_ex", "cd_edge_found",
 ]
 
 SUPPORTED_M = (256, 512, 1024, 2048)
@@ -201,11 +202,25 @@ def precompute_kernel_freq(G, M: int, device="cuda"):
 # ---- drop-in entry points (reference names) --------------------------------
 
 def srkTrainingForward(E, I, Gx_freq, Gy_freq, N, W, erosion, dx, B, ws, R, stream=None):
+    """Batched forward with spectra at size N (reference layout [N, N/2+1]),
+    for any N; the workspace's tile size M may differ from N."""
     img, val = B * N * N, B * W * W
-    call("srkTrainingForward", _dev(E, img, "E"), _dev(I, img, "I"), _spec(Gx_freq, ws, "Gx"),
-         _spec(Gy_freq, ws, "Gy"), N, W, erosion, float(dx), B, _stream(stream),
-         _dev(R, val, "R"), ws.handle)
+    spec = N * (N // 2 + 1)
+    call("srkTrainingForward", _dev(E, img, "E"), _dev(I, img, "I"),
+         _dev(Gx_freq, spec, "Gx", "complex64"), _dev(Gy_freq, spec, "Gy", "complex64"), N, W,
+         erosion, float(dx), B, _stream(stream), _dev(R, val, "R"), ws.handle)
     return R
+
+
+def check_spectra_support(Gx_freq, Gy_freq, Ns, erosion, ws, stream=None) -> float:
+    """srkCheckSpectraSupport: relative RMS of the size-Ns spectra's spatial
+    kernel outside the erosion box; raises SrkError (SRK_ERR_KERNEL) above 1e-4."""
+    r = ctypes.c_double(-1.0)
+    spec = Ns * (Ns // 2 + 1)
+    call("srkCheckSpectraSupport", _dev(Gx_freq, spec, "Gx", "complex64"),
+         _dev(Gy_freq, spec, "Gy", "complex64"), Ns, erosion, _stream(stream), ws.handle,
+         ctypes.byref(r))
+    return r.value
 
 
 def srkForwardOaS(E, I, Gx_freq, Gy_freq, N, W, erosion, dx, ws, R, ux=None, uy=None,
@@ -340,9 +355,55 @@ def tangent_adjoint_batched(f, g, df, dg, lam_h, dlam_h, Gx_freq, Gy_freq, erosi
 
 # ---- critical-dimension measurement ------------------------------------------
 
+FLT_MAX = 3.4028234663852886e38
+
+
+def cd_edge_found(px, L: int = 16):
+    """True where a CD edge (tensor or array of pixels) was found: a missing edge
+    reads -FLT_MAX / L or +FLT_MAX / L."""
+    return abs(px) < 0.5 * FLT_MAX / L
+
+
+def build_polyphase_table(L: int = 16, A: int = 8, stream=None):
+    """BuildPolyphaseTable: [L, 2A+1] float32 device table, rows summing to 1."""
+    torch = _torch()
+    out = torch.empty((L, 2 * A + 1), dtype=torch.float32, device="cuda")
+    call("srkBuildPolyphaseTable", L, A, _ptr(out), _stream(stream))
+    return out
+
+
+def upsample_batched(sig, coeff, L: int = 16, A: int = 8, stream=None):
+    """UpsampleBatched: [B, N] device signals -> [B, N*L] with the table coeff."""
+    torch = _torch()
+    sig = sig.contiguous()
+    B, N = sig.shape
+    out = torch.empty((B, N * L), dtype=torch.float32, device=sig.device)
+    call("srkUpsampleBatched", _dev(sig, B * N, "signals"), _dev(coeff, L * (2 * A + 1), "coeff"),
+         B, N, L, A, _ptr(out), _stream(stream))
+    return out
+
+
+def measure_cd_batched_ex(sig=None, coeff=None, up=None, N=None, L: int = 16, A: int = 8,
+                          stream=None):
+    """MeasureCD_Batched overload with a prebuilt table and / or upsampled
+    signals (zero crossings); sig [B, N], up [B, N*L] (N given or taken from sig)."""
+    torch = _torch()
+    if sig is not None:
+        sig = sig.contiguous()
+        B, N = sig.shape
+    else:
+        B = up.shape[0]
+    out = [torch.empty(B, dtype=torch.float32, device="cuda") for _ in range(3)]
+    call("srkMeasureCDBatchedEx", _dev(sig, B * N, "signals", optional=True),
+         _dev(coeff, L * (2 * A + 1), "coeff", optional=True),
+         _dev(up, B * N * L, "upsampled", optional=True), B, N, L, A, *map(_ptr, out),
+         _stream(stream))
+    return tuple(out)
+
+
 def measure_cd_batched(sig, L: int = 16, A: int = 8, threshold: float = 0.0, stream=None):
     """MeasureCD_Batched of [B, N] row-major device signals: (left_px, right_px,
-    cd_px) device tensors [B]; a missing edge is -inf / +inf."""
+    cd_px) device tensors [B]; a missing edge reads -FLT_MAX / L or +FLT_MAX / L."""
     torch = _torch()
     sig = sig.contiguous()
     B, N = sig.shape

@@ -35,6 +35,7 @@ SIGNATURES = {
     "srkWorkspaceKernelTimes": (_i, [_p, _p, _p, _p]),
     "srkKernelKindName": (ctypes.c_char_p, [_i]),
     "srkTrainingForward": (_i, [_p, _p, _p, _p, _i, _i, _i, _f, _i, _p, _p, _p]),
+    "srkCheckSpectraSupport": (_i, [_p, _p, _i, _i, _p, _p, ctypes.POINTER(_d)]),
     "srkForwardOaS": (_i, [_p, _p, _p, _p, _i, _i, _i, _f, _p, _p, _p, _p, _p]),
     "adjointSRK": (_i, [_p, _p, _p, _p, _p, _p, _i, _i, _i, _f, _p, _p, _p, _p]),
     "shrinkageTangentOaS": (_i, [_p, _p, _p, _p, _p, _p, _i, _i, _i, _f, _p, _p, _p]),
@@ -48,6 +49,9 @@ SIGNATURES = {
     "srkTangentAdjointBatched": (_i, [_p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _f, _i, _p,
                                       _p, _p, _p]),
     "srkMeasureCDBatched": (_i, [_p, _i, _i, _i, _i, _f, _p, _p, _p, _p]),
+    "srkMeasureCDBatchedEx": (_i, [_p, _p, _p, _i, _i, _i, _i, _p, _p, _p, _p]),
+    "srkBuildPolyphaseTable": (_i, [_i, _i, _p, _p]),
+    "srkUpsampleBatched": (_i, [_p, _p, _i, _i, _i, _i, _p, _p]),
     "srkGpGenerate": (_i, [_d, _d, _d, _d, _i, _i, ctypes.c_ulonglong, _i, _p]),
     "srkShrinkageCDBatched": (_i, [_p, _p, _i, _i, _i, _i, _f, _i, _i, _i, _f, _p, _p, _p, _p]),
 }

@@ -267,6 +267,16 @@ struct srk_workspace_s {
     size_t x1_bytes = 0, x2_bytes = 0;
     float2* GT = nullptr;
     float2* tw = nullptr;
+    // spectra handed over at another size (srkTrainingForward with N != M,
+    // srkCheckSpectraSupport): FP64 scratch and the re-transformed spectra
+    void* rs_scratch = nullptr;
+    float2* rs_g = nullptr;  // [2][M][M/2+1], reference layout
+    int rs_ns_max = 0;
+    // completion of the last call (recorded on its stream): a call on another
+    // stream waits for it, since calls share the exchange buffers and spectra
+    hipEvent_t done = nullptr;
+    hipStream_t last = nullptr;
+    bool has_last = false;
     size_t total = 0;
 };
 
@@ -275,6 +285,7 @@ namespace {
 void ws_free(srk_workspace_s* w)
 {
     if (!w) return;
+    if (w->done) hipEventDestroy(w->done);
     if (w->fork.join) hipEventDestroy(w->fork.join);
     if (w->fork.fork) hipEventDestroy(w->fork.fork);
     if (w->fork.aux) hipStreamDestroy(w->fork.aux);
@@ -284,6 +295,8 @@ void ws_free(srk_workspace_s* w)
     hipFree(w->X2);
     hipFree(w->GT);
     hipFree(w->tw);
+    hipFree(w->rs_scratch);
+    hipFree(w->rs_g);
     delete w;
 }
 
@@ -358,11 +371,69 @@ int check_spec(const void* gx, const void* gy)
         if (_s != SRK_OK) return _s;  \
     } while (0)
 
+bool capturing(hipStream_t s)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone;
+}
+
+// Calls on one workspace share its exchange buffers, spectra and scratch.  A
+// call on a stream other than the previous call's waits for that call's
+// completion event first, and every call records the event at its end, so
+// calls issued from one host thread on several streams do not race.  Calls
+// being captured into a graph are left out (their order is the graph's).
+int ws_order(srk_workspace_s* w, hipStream_t s)
+{
+    if (capturing(s)) return SRK_OK;
+    if (w->has_last && w->last != s) SRK_TRY(hip_status(hipStreamWaitEvent(s, w->done, 0)));
+    return SRK_OK;
+}
+
+int ws_record(srk_workspace_s* w, hipStream_t s)
+{
+    if (capturing(s)) return SRK_OK;
+    SRK_TRY(hip_status(hipEventRecord(w->done, s)));
+    w->last = s;
+    w->has_last = true;
+    return SRK_OK;
+}
+
 int import_spectra(srk_workspace_s* w, const void* gx, const void* gy, float dx, hipStream_t s)
 {
+    SRK_TRY(ws_order(w, s));
     const float scale = (float)((double)dx * dx / ((double)w->M * w->M));
     return hip_status(srk::engine_import_spectra(w->M, (const float2*)gx, (const float2*)gy,
                                                  scale, w->GT, s));
+}
+
+// Spectra of size Ns (reference layout) -> the kernel on the erosion box ->
+// spectra at the workspace's tile size in w->rs_g (when `resample`).  Unless
+// the stream is being captured, waits for the support measure and returns
+// SRK_ERR_KERNEL when the Ns-periodic kernel reaches beyond the box |x|, |y|
+// <= erosion (relative RMS outside above 1e-4): the valid region would then
+// see taps the erosion does not cover.  *ratio_out (optional) receives that
+// measure (-1 while capturing).
+constexpr double kSupportTol = 1e-4;
+
+int convert_spectra(srk_workspace_s* w, const void* gx, const void* gy, int Ns, bool resample,
+                    hipStream_t s, double* ratio_out)
+{
+    if (Ns < 2 * w->e + 1 || Ns > w->rs_ns_max) return SRK_ERR_UNSUPPORTED_SIZE;
+    SRK_TRY(ws_order(w, s));
+    double* d_ratio = nullptr;
+    float2* dst = resample ? w->rs_g : nullptr;
+    const size_t plane = (size_t)w->M * (w->M / 2 + 1);
+    SRK_TRY(hip_status(srk::resample_spectra_device(
+        Ns, w->e, w->M, (const float2*)gx, (const float2*)gy, w->rs_scratch, dst,
+        dst ? dst + plane : nullptr, &d_ratio, w->rs_ns_max, s)));
+    if (ratio_out) *ratio_out = -1.0;
+    if (capturing(s)) return SRK_OK;
+    SRK_TRY(ws_record(w, s));
+    double r = 0.0;
+    SRK_TRY(hip_status(hipMemcpyAsync(&r, d_ratio, sizeof(double), hipMemcpyDeviceToHost, s)));
+    SRK_TRY(hip_status(hipStreamSynchronize(s)));
+    if (ratio_out) *ratio_out = r;
+    return r <= kSupportTol ? SRK_OK : SRK_ERR_KERNEL;
 }
 
 // The launch timer to use on stream s: none unless timing is on and s is not
@@ -396,8 +467,9 @@ int run_fwd(srk_workspace_s* w, srk::Op op, srk::EngineArgs a, int B, hipStream_
     a.halo2 = 0;
     const int ntiles = B * w->nft;
     const int chunk = chunk_for(w, a.np1, a.np2, a.S2, ntiles);
-    return hip_status(srk::engine_run(op, w->M, w->precision, a, ntiles, chunk, s,
-                                      timer_for(w, s), lanes_for(w, a)));
+    SRK_TRY(hip_status(srk::engine_run(op, w->M, w->precision, a, ntiles, chunk, s,
+                                       timer_for(w, s), lanes_for(w, a))));
+    return ws_record(w, s);
 }
 
 int run_adj(srk_workspace_s* w, srk::Op op, srk::EngineArgs a, int B, hipStream_t s)
@@ -413,8 +485,9 @@ int run_adj(srk_workspace_s* w, srk::Op op, srk::EngineArgs a, int B, hipStream_
     a.halo2 = 1;
     const int ntiles = B * w->nat;
     const int chunk = chunk_for(w, a.np1, a.np2, a.S2, ntiles);
-    return hip_status(srk::engine_run(op, w->M, w->precision, a, ntiles, chunk, s,
-                                      timer_for(w, s), lanes_for(w, a)));
+    SRK_TRY(hip_status(srk::engine_run(op, w->M, w->precision, a, ntiles, chunk, s,
+                                       timer_for(w, s), lanes_for(w, a))));
+    return ws_record(w, s);
 }
 
 }  // namespace
@@ -448,7 +521,7 @@ int srkPrecomputeKernelFreq(const float* h_G, int kw, int kh, int M, void* d_G_f
 {
     if (!h_G || !d_G_freq) return SRK_ERR_NULL_PTR;
     if (kw < 1 || kh < 1 || kw % 2 == 0 || kh % 2 == 0 || kw > M || kh > M) return SRK_ERR_KERNEL;
-    if (M < 2 || (M & (M - 1))) return SRK_ERR_UNSUPPORTED_SIZE;
+    if (M < 2) return SRK_ERR_UNSUPPORTED_SIZE;  // any size, as cuFFT plans any N
     std::vector<double> G((size_t)kw * kh);
     for (size_t i = 0; i < G.size(); ++i) G[i] = h_G[i];
     std::vector<float2> out;
@@ -462,7 +535,7 @@ int srkKernelSpectraSRSRK(int kw, double sigma, double dx, double cw, int M, voi
                           void* d_Gy_freq, float* h_Gx, float* h_Gy, hipStream_t stream)
 {
     if (kw < 3 || kw % 2 == 0 || !(sigma > 0) || !(dx > 0) || !(cw > 0)) return SRK_ERR_KERNEL;
-    if (M < 2 || (M & (M - 1)) || kw > M) return SRK_ERR_UNSUPPORTED_SIZE;
+    if (M < 2 || kw > M) return SRK_ERR_UNSUPPORTED_SIZE;
     const SpecKey key{kw, sigma, dx, cw, M};
     SpecVal* val = nullptr;
     {
@@ -497,7 +570,7 @@ int srkKernelSpectraSRSRKDevice(int kw, double sigma, double dx, double cw, int 
                                 hipStream_t stream)
 {
     if (kw < 3 || kw % 2 == 0 || !(sigma > 0) || !(dx > 0) || !(cw > 0)) return SRK_ERR_KERNEL;
-    if (M < 2 || (M & (M - 1)) || kw > M) return SRK_ERR_UNSUPPORTED_SIZE;
+    if (M < 2 || kw > M) return SRK_ERR_UNSUPPORTED_SIZE;
     if (!d_Gx_freq || !d_Gy_freq) return SRK_ERR_NULL_PTR;
     if (!aligned((const char*)d_Gx_freq, 8) || !aligned((const char*)d_Gy_freq, 8))
         return SRK_ERR_MISALIGNED;
@@ -577,11 +650,15 @@ int srkWorkspaceCreate(int N, int W, int erosion, int M, int B_max, size_t scrat
     alloc((void**)&w->X2, w->x2_bytes);
     alloc((void**)&w->GT, (size_t)2 * (M / 2 + 1) * M * sizeof(float2));
     alloc((void**)&w->tw, tw.size() * sizeof(float2));
+    w->rs_ns_max = std::max(N, M);
+    alloc(&w->rs_scratch, srk::resample_scratch_bytes(w->rs_ns_max, erosion, M));
+    alloc((void**)&w->rs_g, (size_t)2 * M * (M / 2 + 1) * sizeof(float2));
     if (st == SRK_OK) st = hip_status(hipStreamCreateWithFlags(&w->fork.aux, hipStreamNonBlocking));
     if (st == SRK_OK)
         st = hip_status(hipEventCreateWithFlags(&w->fork.fork, hipEventDisableTiming));
     if (st == SRK_OK)
         st = hip_status(hipEventCreateWithFlags(&w->fork.join, hipEventDisableTiming));
+    if (st == SRK_OK) st = hip_status(hipEventCreateWithFlags(&w->done, hipEventDisableTiming));
     if (st == SRK_OK)
         st = hip_status(hipMemcpy(w->d_ftiles, fb.data(), fb.size() * sizeof(srk::Tile), hipMemcpyHostToDevice));
     if (st == SRK_OK)
@@ -700,6 +777,15 @@ int srkForwardBatched(const float* E, const float* I, const void* Gx, const void
     return run_fwd(w, srk::Op::Forward, a, B, stream);
 }
 
+int srkCheckSpectraSupport(const void* Gx, const void* Gy, int Ns, int erosion,
+                           hipStream_t stream, srk_workspace_t w, double* ratio)
+{
+    if (!w) return SRK_ERR_NULL_PTR;
+    SRK_TRY(check_spec(Gx, Gy));
+    if (erosion != w->e) return SRK_ERR_WORKSPACE;
+    return convert_spectra(w, Gx, Gy, Ns, false, stream, ratio);
+}
+
 int srkForwardBatchedU(const float* E, const float* I, const void* Gx, const void* Gy, int N,
                        int W, int erosion, float dx, int B, hipStream_t stream, float* R,
                        float* ux, float* uy, srk_workspace_t w)
@@ -733,8 +819,19 @@ int srkTrainingForward(const float* E, const float* I, const void* Gx, const voi
                        int W, int erosion, float dx, int B, hipStream_t stream, float* R,
                        srk_workspace_t w)
 {
-    if (w && w->M != N) return SRK_ERR_WORKSPACE;  // spectra are at size N
-    return srkForwardBatched(E, I, Gx, Gy, N, W, erosion, dx, B, stream, R, w);
+    if (w && w->M == N) return srkForwardBatched(E, I, Gx, Gy, N, W, erosion, dx, B, stream, R, w);
+    // spectra at size N, tiles of size M != N: the spectra fix the spatial
+    // kernel on the erosion box exactly (its support must lie inside it, as
+    // for the reference's circular transform at size N to be exact on the
+    // valid region); re-transform it at size M
+    SRK_TRY(check_ws(w, N, W, erosion, B));
+    SRK_TRY(check_f(E)); SRK_TRY(check_f(I)); SRK_TRY(check_f(R)); SRK_TRY(check_spec(Gx, Gy));
+    SRK_TRY(convert_spectra(w, Gx, Gy, N, true, stream, nullptr));
+    const size_t plane = (size_t)w->M * (w->M / 2 + 1);
+    SRK_TRY(import_spectra(w, w->rs_g, w->rs_g + plane, dx, stream));
+    srk::EngineArgs a = base_args(w, dx);
+    a.g0 = E; a.f0 = I; a.out0 = R;
+    return run_fwd(w, srk::Op::Forward, a, B, stream);
 }
 
 int srkForwardOaS(const float* E, const float* I, const void* Gx, const void* Gy, int N, int W,
@@ -837,6 +934,43 @@ int srkMeasureCDBatched(const float* d_in, int B, int N, int L, int A, float thr
     if (!srk::cd_shape_ok(N, L, A)) return SRK_ERR_UNSUPPORTED_SIZE;
     return hip_status(srk::cd_measure_rows(d_in, B, N, L, A, threshold, d_left_px, d_right_px,
                                            d_cd_px, stream));
+}
+
+int srkBuildPolyphaseTable(int L, int A, float* d_coeff, hipStream_t stream)
+{
+    if (!d_coeff) return SRK_ERR_NULL_PTR;
+    if (!aligned(d_coeff, sizeof(float))) return SRK_ERR_MISALIGNED;
+    if (L < 1 || A < 1 || L > 4096 || A > 256) return SRK_ERR_BAD_DIMS;
+    return hip_status(srk::cd_build_table(L, A, d_coeff, stream));
+}
+
+int srkUpsampleBatched(const float* d_in, const float* d_coeff, int B, int N, int L, int A,
+                       float* d_out, hipStream_t stream)
+{
+    if (!d_in || !d_coeff || !d_out) return SRK_ERR_NULL_PTR;
+    if (!aligned(d_in, sizeof(float)) || !aligned(d_coeff, sizeof(float)) ||
+        !aligned(d_out, sizeof(float)))
+        return SRK_ERR_MISALIGNED;
+    if (B < 1) return SRK_ERR_BATCH;
+    if (N < 1 || L < 1 || A < 1 || L > 4096 || A > 256) return SRK_ERR_BAD_DIMS;
+    if ((long long)N * L > (1LL << 31) - 1) return SRK_ERR_UNSUPPORTED_SIZE;
+    return hip_status(srk::cd_upsample(d_in, d_coeff, B, N, L, A, d_out, stream));
+}
+
+int srkMeasureCDBatchedEx(const float* d_in, const float* d_coeff, const float* d_up, int B, int N,
+                          int L, int A, float* d_left_px, float* d_right_px, float* d_cd_px,
+                          hipStream_t stream)
+{
+    if ((!d_in && !d_up) || !d_left_px || !d_right_px || !d_cd_px) return SRK_ERR_NULL_PTR;
+    if (B < 1) return SRK_ERR_BATCH;
+    if (N < 2 || L < 1 || A < 1) return SRK_ERR_BAD_DIMS;
+    if (d_up) {
+        if ((long long)N * L > (1LL << 24)) return SRK_ERR_UNSUPPORTED_SIZE;
+    } else if (!srk::cd_shape_ok(N, L, A)) {
+        return SRK_ERR_UNSUPPORTED_SIZE;
+    }
+    return hip_status(srk::cd_measure_ex(d_in, d_coeff, d_up, B, N, L, A, d_left_px, d_right_px,
+                                         d_cd_px, stream));
 }
 
 int srkShrinkageCDBatched(const float* d_E, const float* d_R, int N, int W, int erosion, int B,

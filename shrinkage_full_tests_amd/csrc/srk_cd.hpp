@@ -23,4 +23,15 @@ int cd_measure_shrink(const float* E, const float* R, int N, int W, int e, int B
                       int center_row, int L, int A, float thr, float* left_px, float* right_px,
                       float* cd_px, hipStream_t stream);
 
+// Polyphase table coeff [L][2A+1] (rows normalised to sum 1), and the batched
+// upsampling out [B][N L] of row-major signals with a given table.
+int cd_build_table(int L, int A, float* coeff, hipStream_t stream);
+int cd_upsample(const float* in, const float* coeff, int B, int N, int L, int A, float* out,
+                hipStream_t stream);
+
+// Zero-crossing CD from the caller's table (coeff, or null to build it) and /
+// or upsampled signals (up [B][N L], or null to upsample `in` on the fly).
+int cd_measure_ex(const float* in, const float* coeff, const float* up, int B, int N, int L,
+                  int A, float* left_px, float* right_px, float* cd_px, hipStream_t stream);
+
 }  // namespace srk

@@ -1021,23 +1021,26 @@ constexpr size_t p2_lds()
     return (p2_one_line<M, PL, PASS>() ? 1ull : 2ull) * RY * p2_pitch<M, RY>() * sizeof(float2);
 }
 
+// The dynamic-LDS limits are per-kernel attributes: set once per process
+// (a function-local static is initialised exactly once, also under
+// concurrent first calls).
 template <int M, int RY, class PL, class SX>
 hipError_t prepare_kernels()
 {
-    static bool done = false;
-    static hipError_t st = hipSuccess;
-    if (done) return st;
-    constexpr int NW = P2Waves<M, RY, PL>::NW;
-    constexpr int lc = (int)p2_lds<M, RY, PL, kP2ConvPass>();
-    constexpr int lr = (int)p2_lds<M, RY, PL, kP2CorrPass>();
-    const struct { const void* fn; int lds; } fns[] = {
-        {reinterpret_cast<const void*>(&k_p2_conv<M, RY, NW, PL, float2>), lc},
-        {reinterpret_cast<const void*>(&k_p2_conv<M, RY, NW, PL, SX>), lc},
-        {reinterpret_cast<const void*>(&k_p2_corr<M, RY, NW, PL, SX>), lr}};
-    for (const auto& f : fns)
-        if (st == hipSuccess)
-            st = hipFuncSetAttribute(f.fn, hipFuncAttributeMaxDynamicSharedMemorySize, f.lds);
-    done = true;
+    static const hipError_t st = [] {
+        constexpr int NW = P2Waves<M, RY, PL>::NW;
+        constexpr int lc = (int)p2_lds<M, RY, PL, kP2ConvPass>();
+        constexpr int lr = (int)p2_lds<M, RY, PL, kP2CorrPass>();
+        const struct { const void* fn; int lds; } fns[] = {
+            {reinterpret_cast<const void*>(&k_p2_conv<M, RY, NW, PL, float2>), lc},
+            {reinterpret_cast<const void*>(&k_p2_conv<M, RY, NW, PL, SX>), lc},
+            {reinterpret_cast<const void*>(&k_p2_corr<M, RY, NW, PL, SX>), lr}};
+        hipError_t e = hipSuccess;
+        for (const auto& f : fns)
+            if (e == hipSuccess)
+                e = hipFuncSetAttribute(f.fn, hipFuncAttributeMaxDynamicSharedMemorySize, f.lds);
+        return e;
+    }();
     return st;
 }
 
@@ -1172,13 +1175,15 @@ int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s0,
             p3a(2, 1, 4);
             break;
         }
-        if (st != hipSuccess) return (int)st;
-        st = hipGetLastError();
-        if (st != hipSuccess) return (int)st;
+        if (st == hipSuccess) st = hipGetLastError();
+        if (st != hipSuccess) break;
     }
+    // the caller's stream always ends after the second lane's work, on the
+    // error path too (a capture must not be left with an unjoined stream)
     if (aux_used) {
-        st = hipEventRecord(fk->join, fk->aux);
-        if (st == hipSuccess) st = hipStreamWaitEvent(s0, fk->join, 0);
+        const hipError_t j1 = hipEventRecord(fk->join, fk->aux);
+        const hipError_t j2 = j1 == hipSuccess ? hipStreamWaitEvent(s0, fk->join, 0) : j1;
+        if (st == hipSuccess) st = j2;
     }
     return (int)st;
 }

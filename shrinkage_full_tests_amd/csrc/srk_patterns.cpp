@@ -18,7 +18,9 @@
 
 #include <cmath>
 #include <complex>
+#include <algorithm>
 #include <random>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -57,19 +59,37 @@ void fft_inplace(cd* a, int n, const std::vector<cd>& tw, bool inverse)
         for (int i = 0; i < n; ++i) a[i] /= n;
 }
 
+// run fn(i) for i in [0, n) on up to 16 host threads (lines are independent,
+// so the result does not depend on the thread count)
+template <class Fn>
+void parallel_lines(int n, Fn fn)
+{
+    const int T = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (T == 1 || n < 64) {
+        for (int i = 0; i < n; ++i) fn(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; ++t)
+        th.emplace_back([&, t]() {
+            for (int i = t; i < n; i += T) fn(i);
+        });
+    for (auto& x : th) x.join();
+}
+
 // 2-D FFT of a row-major w x h grid: rows, then columns
 void fft2(std::vector<cd>& d, int w, int h, bool inverse)
 {
     const int N = std::max(w, h);
     std::vector<cd> tw(N);
     for (int k = 0; k < N; ++k) tw[k] = std::polar(1.0, -2.0 * M_PI * k / N);
-    for (int r = 0; r < h; ++r) fft_inplace(d.data() + (size_t)r * w, w, tw, inverse);
-    std::vector<cd> col(h);
-    for (int c = 0; c < w; ++c) {
+    parallel_lines(h, [&](int r) { fft_inplace(d.data() + (size_t)r * w, w, tw, inverse); });
+    parallel_lines(w, [&](int c) {
+        std::vector<cd> col(h);
         for (int r = 0; r < h; ++r) col[r] = d[(size_t)r * w + c];
         fft_inplace(col.data(), h, tw, inverse);
         for (int r = 0; r < h; ++r) d[(size_t)r * w + c] = col[r];
-    }
+    });
 }
 
 }  // namespace

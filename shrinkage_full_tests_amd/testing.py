@@ -157,15 +157,20 @@ def _err(a: float, n: float, tol_abs: float, tol_rel: float):
     return ae, re, ae <= max(tol_abs, tol_rel * scale)
 
 
-def _sample(shape, num_samples: int, seed: int):
+def _sample_index(shape, num_samples: int, seed: int):
+    """Flat indices (sorted) of the pixels to check in a (W, H) buffer; all
+    pixels when num_samples < 0 or >= W*H."""
     W, H = shape[-2], shape[-1]
     total = W * H
     if num_samples < 0 or num_samples >= total:
-        idx = np.arange(total)
-    else:
-        rng = np.random.default_rng(seed if seed else None)
-        idx = np.sort(rng.choice(total, size=num_samples, replace=False))
-    return [(int(i // H), int(i % H)) for i in idx]
+        return np.arange(total)
+    rng = np.random.default_rng(seed if seed else None)
+    return np.sort(rng.choice(total, size=num_samples, replace=False))
+
+
+def _sample(shape, num_samples: int, seed: int):
+    H = shape[-1]
+    return [(int(i // H), int(i % H)) for i in _sample_index(shape, num_samples, seed)]
 
 
 # ---- adjoint ------------------------------------------------------------------
@@ -197,6 +202,11 @@ class AdjointResult:
 
 
 def _q(b) -> float:
+    """q = sum b^2 in float64 (on the device for device tensors, so large
+    outputs are not copied to the host per perturbed pixel)."""
+    if hasattr(b, "detach") and b.is_cuda:
+        x = b.detach().double()
+        return float((x * x).sum())
     x = _host(b).astype(np.float64)
     return float((x * x).sum())
 
@@ -263,37 +273,54 @@ class EpsilonResult:
 @dataclasses.dataclass
 class TangentResult:
     per_epsilon: List[EpsilonResult]
-    pixels: List[PixelResult]      # at the best epsilon
     passed: bool
     best_epsilon: float
     best_abs_error_max: float
     best_rel_error_max: float
     best_num_violations: int
     num_samples: int
+    # per-pixel data at the best epsilon (flat pixel index, analytical,
+    # numerical), kept as arrays so that full-image checks of large images
+    # stay cheap; `pixels` builds the PixelResult list on demand
+    _flat: np.ndarray = dataclasses.field(default=None, repr=False)
+    _ana: np.ndarray = dataclasses.field(default=None, repr=False)
+    _num: np.ndarray = dataclasses.field(default=None, repr=False)
+    _H: int = 1
+
+    @property
+    def pixels(self) -> List[PixelResult]:
+        out = []
+        for i, a, n in zip(self._flat, self._ana, self._num):
+            ae = abs(float(a) - float(n))
+            sc = max(abs(float(a)), abs(float(n)))
+            out.append(PixelResult(int(i // self._H), int(i % self._H), float(a), float(n),
+                                   ae / sc if sc > 0 else 0.0, ae))
+        return out
 
 
 def _directional_check(fn: Callable, state: Sequence, dirs: Sequence, analytical,
                        cfg: TangentConfig) -> TangentResult:
     ana = _host(analytical).astype(np.float64)
-    pix = _sample(ana.shape, cfg.num_samples, cfg.seed)
-    sel = (np.array([p[0] for p in pix]), np.array([p[1] for p in pix]))
-    per, pixel_sets = [], []
+    H = ana.shape[-1]
+    flat = _sample_index(ana.shape, cfg.num_samples, cfg.seed)
+    full = flat.size == ana.shape[-2] * H
+    a = ana.reshape(-1) if full else ana.reshape(-1)[flat]
+    per, nums = [], []
     for eps in cfg.epsilons:
         sp = [s + eps * d if d is not None else s for s, d in zip(state, dirs)]
         sm = [s - eps * d if d is not None else s for s, d in zip(state, dirs)]
         num = (_host(fn(*sp)).astype(np.float64) - _host(fn(*sm)).astype(np.float64)) / (2 * eps)
-        a = ana[..., sel[0], sel[1]].reshape(-1)
-        n = num[..., sel[0], sel[1]].reshape(-1)
+        n = num.reshape(-1) if full else num.reshape(-1)[flat]
+        del num
         ae = np.abs(a - n)
         scale = np.maximum(np.abs(a), np.abs(n))
-        re = np.where(scale > 0, ae / np.where(scale > 0, scale, 1.0), 0.0)
+        re = np.divide(ae, scale, out=np.zeros_like(ae), where=scale > 0)
         bad = ae > np.maximum(cfg.tol_abs, cfg.tol_rel * scale)
         w = int(np.argmax(ae)) if ae.size else 0
         per.append(EpsilonResult(eps, int(bad.sum()), float(ae.max(initial=0.0)),
-                                 float(re.max(initial=0.0)), pix[w][0], pix[w][1],
-                                 not bad.any()))
-        pixel_sets.append([PixelResult(px, py, float(aa), float(nn), float(rr), float(ee))
-                           for (px, py), aa, nn, rr, ee in zip(pix, a, n, re, ae)])
+                                 float(re.max(initial=0.0)), int(flat[w] // H) if ae.size else 0,
+                                 int(flat[w] % H) if ae.size else 0, not bad.any()))
+        nums.append(n)
     best, any_pass = 0, False
     for k, er in enumerate(per):
         if er.passed:
@@ -305,10 +332,10 @@ def _directional_check(fn: Callable, state: Sequence, dirs: Sequence, analytical
                      er.abs_error_max < per[best].abs_error_max)):
                 best = k
     b = per[best]
-    return TangentResult(per_epsilon=per, pixels=pixel_sets[best], passed=any_pass,
-                         best_epsilon=b.epsilon, best_abs_error_max=b.abs_error_max,
-                         best_rel_error_max=b.rel_error_max,
-                         best_num_violations=b.num_violations, num_samples=len(pix))
+    return TangentResult(per_epsilon=per, passed=any_pass, best_epsilon=b.epsilon,
+                         best_abs_error_max=b.abs_error_max, best_rel_error_max=b.rel_error_max,
+                         best_num_violations=b.num_violations, num_samples=int(flat.size),
+                         _flat=flat, _ana=a, _num=nums[best], _H=H)
 
 
 def tangent_test(forward: Callable, tangent: Callable, state: Sequence, dirs: Sequence,

@@ -75,7 +75,9 @@ def test_kernel_spectra_argument_errors():
     L = _lib.lib()
     assert L.srkKernelSpectraSRSRK(30, 5.0, 1.0, 3.0, 256, None, None, None, None, None) == -9
     assert L.srkKernelSpectraSRSRK(31, -1.0, 1.0, 3.0, 256, None, None, None, None, None) == -9
-    assert L.srkKernelSpectraSRSRK(31, 5.0, 1.0, 3.0, 300, None, None, None, None, None) == -5
+    # any spectrum size (the reference plans cuFFT at any N), but not below the kernel width
+    assert L.srkKernelSpectraSRSRK(31, 5.0, 1.0, 3.0, 16, None, None, None, None, None) == -5
+    assert L.srkKernelSpectraSRSRK(31, 5.0, 1.0, 3.0, 300, None, None, None, None, None) == 0
     assert L.srkPrecomputeKernelFreq(None, 31, 31, 256, None, None) == -3
 
 

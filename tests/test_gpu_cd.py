@@ -38,7 +38,7 @@ def _run(orc, imgs, vert, kw, sigma, coeff_c, threshold):
         sig_r.append(sr[:, W // 2] if vertical else sr[W // 2, :])
     _, _, cg = cdm.measure_cd(np.stack(sig_g), threshold=threshold)
     _, _, cr = cdm.measure_cd(np.stack(sig_r), threshold=threshold)
-    assert np.all(np.isfinite(cr))
+    assert np.all(cdm.found(cr))
     return np.abs(cg - cr), cr
 
 

@@ -29,10 +29,14 @@ def host(t):
 def _check(got, sig, L, A, thr):
     # the reference sample is FP32: compare with the oracle on the FP32 values
     ref = cdm.measure_cd(np.asarray(sig, np.float32).astype(np.float64), L, A, thr)
+    both = cdm.found(ref[0], L) & cdm.found(ref[1], L)
     for g, r, name in zip(got, ref, ("left", "right", "cd")):
         g = host(g)
-        fin = np.isfinite(r)
-        assert np.array_equal(np.isfinite(g), fin), (name, g, r)
+        fin = both if name == "cd" else cdm.found(r, L)
+        assert np.array_equal(cdm.found(g, L) if name != "cd" else
+                              cdm.found(host(got[0]), L) & cdm.found(host(got[1]), L), fin), (
+            name, g, r)
+        # missing edges: the reference's FP32 values -/+FLT_MAX/L, bit for bit
         assert np.all(g[~fin] == r[~fin]), (name, g, r)
         assert np.abs(g[fin] - r[fin]).max(initial=0.0) < TOL_PX, (name, g, r)
     return ref
@@ -53,7 +57,7 @@ def test_measure_cd_smooth_signals(N, L, A):
         sigs.append(line + 0.02 * rng.standard_normal(N))
     sig = np.stack(sigs)
     ref = _check(srk.measure_cd_batched(dev(sig), L, A, threshold=0.5), sig, L, A, 0.5)
-    assert np.all(np.isfinite(ref[2]))
+    assert np.all(cdm.found(ref[0], L) & cdm.found(ref[1], L))
 
 
 def test_measure_cd_gaussian_fwhm():
@@ -70,8 +74,38 @@ def test_measure_cd_missing_edges_and_zero_threshold():
     x = np.arange(N)
     sig = np.stack([np.ones(N),                          # no crossing at all
                     np.where(x < 80, -1.0, 1.0),         # right edge only
+                    np.where(x < 40, 1.0, -1.0),         # left edge only
                     np.sin(2 * np.pi * (x - 64) / 50.0)])  # zero crossings (threshold 0)
-    _check(srk.measure_cd_batched(dev(sig), 16, 8, threshold=0.0), sig, 16, 8, 0.0)
+    for L in (16, 5):
+        got = srk.measure_cd_batched(dev(sig), L, 8, threshold=0.0)
+        _check(got, sig, L, 8, 0.0)
+        # the reference's formula (cd_measure_util.cu:37,264-282) in FP32, bit for bit
+        inv = np.float32(1.0) / np.float32(L)
+        lmiss, rmiss = np.float32(-3.402823466e+38) * inv, np.float32(3.402823466e+38) * inv
+        l, r, c = (t.cpu().numpy() for t in got)
+        assert l[0] == lmiss and r[0] == rmiss and c[0] == np.float32(rmiss - lmiss)
+        assert l[1] == lmiss and np.isfinite(c[1]) and c[1] == np.float32(r[1] - lmiss)
+        assert r[2] == rmiss and c[2] == np.float32(rmiss - l[2])
+
+
+def test_measure_cd_prebuilt_table_and_upsampled_signals():
+    """BuildPolyphaseTable / UpsampleBatched / the MeasureCD_Batched overload
+    with a table and upsampled signals (cd_measure_util.cuh:11-33,80-95)."""
+    N, L, A = 200, 16, 8
+    rng = np.random.default_rng(3)
+    x = np.arange(N)
+    sig = np.stack([np.cos(2 * np.pi * (x - 100 + rng.uniform(-3, 3)) / rng.uniform(40, 90))
+                    for _ in range(5)])
+    coeff = srk.build_polyphase_table(L, A)
+    np.testing.assert_allclose(host(coeff), cdm.polyphase_table(L, A), atol=2e-6)
+    up = srk.upsample_batched(dev(sig), coeff, L, A)
+    np.testing.assert_allclose(host(up), cdm.upsample(sig.astype(np.float32), L, A), atol=2e-5)
+    ref = srk.measure_cd_batched(dev(sig), L, A, threshold=0.0)
+    for kw in (dict(coeff=coeff), dict(up=up, N=N), dict(coeff=coeff, up=up, N=N)):
+        got = srk.measure_cd_batched_ex(sig=dev(sig) if "up" not in kw else None, L=L, A=A, **kw)
+        for g, r in zip(got, ref):
+            assert np.abs(host(g) - host(r)).max() < 1e-3, kw
+    _check(srk.measure_cd_batched_ex(sig=dev(sig), coeff=coeff, L=L, A=A), sig, L, A, 0.0)
 
 
 def test_measure_cd_rejects_oversized_signals():
@@ -115,6 +149,6 @@ def test_shrinkage_cd_fused_matches_host_pipeline(orc, kw):
         sig_ref.append(sr[:, W // 2] if b < half else sr[W // 2, :])
     _, _, cd_same = cdm.measure_cd(np.stack(sig_same))
     _, _, cd_ref = cdm.measure_cd(np.stack(sig_ref))
-    assert np.all(np.isfinite(cd_ref))
+    assert np.all(cd_ref < 0.5 * cdm.FLT_MAX / 16)
     assert np.abs(cd_gpu - cd_same).max() < TOL_PX, (cd_gpu, cd_same)
     assert np.abs(cd_gpu - cd_ref).max() < TOL_PX, (cd_gpu, cd_ref)

@@ -106,7 +106,7 @@ def test_mixed_cd_box_lines(orc, prec):
         sr.append(r[:, W // 2] if vert[b] else r[W // 2, :])
     _, _, cg = cdm.measure_cd(np.stack(sg), threshold=0.5)
     _, _, cr = cdm.measure_cd(np.stack(sr), threshold=0.5)
-    assert np.all(np.isfinite(cr)) and np.abs(cg - cr).max() < 0.01, (cg, cr)
+    assert np.all(cdm.found(cr)) and np.abs(cg - cr).max() < 0.01, (cg, cr)
 
 
 def test_mixed_precision_rejected_for_large_tiles():

@@ -246,3 +246,42 @@ def test_window_forms_match_whole_image(orc):
         a, b = orc.adjoint_window(v, f, g, e, Kx, Ky, x0, y0, wx, wy, s=1.3)
         np.testing.assert_allclose(a, lf[x0:x0 + wx, y0:y0 + wy], rtol=1e-11, atol=1e-14)
         np.testing.assert_allclose(b, lg[x0:x0 + wx, y0:y0 + wy], rtol=1e-11, atol=1e-14)
+
+
+def test_tangent_window_forms_match_whole_image(orc):
+    """The window forms of the tangent pair against the whole-image oracle
+    tangent / tangent-adjoint (themselves pinned by exact central differences
+    above); a swapped df / dg or lam_h / dlam_h changes every window."""
+    N, kw, e = 40, 9, 5
+    Kx, Ky = _kernel(orc, kw)
+    f, g, df, dg = _rand_fields(N, 91, 4)
+    W = N - 2 * e
+    rng = np.random.default_rng(9)
+    lam, dlam = rng.standard_normal((W, W)), rng.standard_normal((W, W))
+    dh = orc.tangent(f, g, df, dg, e, Kx, Ky, s=0.7)
+    np.testing.assert_allclose(orc.tangent_window(f, g, df, dg, e, Kx, Ky, 2, 4, 11, 9, s=0.7),
+                               dh[2:13, 4:13], rtol=1e-11, atol=1e-14)
+    a, b = orc.tangent_adjoint(f, g, df, dg, lam, dlam, e, Kx, Ky, s=0.7)
+    for (x0, y0, wx, wy) in ((0, 0, 8, 9), (13, 25, 12, 15)):
+        ta, tb = orc.tangent_adjoint_window(f, g, df, dg, lam, dlam, e, Kx, Ky, x0, y0, wx, wy,
+                                            s=0.7)
+        np.testing.assert_allclose(ta, a[x0:x0 + wx, y0:y0 + wy], rtol=1e-11, atol=1e-14)
+        np.testing.assert_allclose(tb, b[x0:x0 + wx, y0:y0 + wy], rtol=1e-11, atol=1e-14)
+    sa, _ = orc.tangent_adjoint_window(f, g, df, dg, dlam, lam, e, Kx, Ky, 13, 25, 12, 15, s=0.7)
+    assert np.abs(sa - a[13:25, 25:40]).max() > 1e-3 * np.abs(a).max()
+
+
+def test_cd_missing_edges_follow_reference_start_values():
+    """A signal without a crossing on one side keeps the reference's search start
+    value -/+FLT_MAX (cd_measure_util.cu:37,218-219), divided by L in FP32."""
+    from oracle import cd
+    N = 64
+    x = np.arange(N)
+    sig = np.stack([np.ones(N), np.where(x < 50, -1.0, 1.0), np.where(x < 20, 1.0, -1.0)])
+    left, right, w = cd.measure_cd(sig, L=16, A=8)
+    fm = float(np.finfo(np.float32).max)
+    assert left[0] == -fm / 16 and right[0] == fm / 16 and w[0] == fm / 8
+    assert left[1] == -fm / 16 and abs(right[1] - 49.5) < 0.6
+    assert right[2] == fm / 16 and abs(left[2] - 19.5) < 0.6
+    assert list(cd.found(left)) == [False, False, True]
+    assert list(cd.found(w)) == [False, False, False]
