@@ -52,4 +52,16 @@ def test_device_spectra_argument_errors():
     with pytest.raises(srk.SrkError):
         srk.kernel_spectra_srsrk_device(31, -1.0, 256)    # sigma <= 0
     with pytest.raises(srk.SrkError):
-        srk.kernel_spectra_srsrk_device(31, 5.0, 300)     # M not a power of two
+        srk.kernel_spectra_srsrk_device(31, 5.0, 16)      # size below the kernel width
+
+
+def test_device_spectra_any_size_match_host():
+    """Spectra at a size that is not a power of two (the reference plans cuFFT
+    at any N) agree between the device and the host FP64 paths."""
+    import torch
+    gx_h, gy_h = srk.kernel_spectra_srsrk(31, 5.0, 300)
+    gx_d, gy_d = srk.kernel_spectra_srsrk_device(31, 5.0, 300)
+    torch.cuda.synchronize()
+    for h, d in ((gx_h, gx_d), (gy_h, gy_d)):
+        peak = float(h.abs().max())
+        assert float((h - d).abs().max()) <= 1e-6 * peak
