@@ -210,6 +210,22 @@ int srkForwardOaS(const float* E, const float* I, const void* Gx_freq, const voi
                   srk_workspace_t ws, float* ux, float* uy);
 
 /*
+ * No counterpart in the reference.  srkForwardOaS for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): no workspace; tile size M in {256, 512,
+ * 1024} is passed explicitly with spectra at size M; FP32 transforms.  The
+ * tile table, twiddles, imported spectra and exchange buffers live in a
+ * static 64 MiB device array that each GPU the library runs on holds until
+ * the process ends; tiles stream through it a chunk at a time.  A call that
+ * needs more than the array (one tile's exchange planes plus the tables)
+ * fails with SRK_ERR_UNSUPPORTED_SIZE.  Allocates nothing, synchronises
+ * nothing.  Calls that overlap on one GPU share the array and race.
+ */
+int srkForwardOaSNoAlloc(const float* E, const float* I, const void* Gx_freq, const void* Gy_freq,
+                         int N, int W, int erosion, float dx, int M, hipStream_t stream, float* R,
+                         float* ux, float* uy);
+
+/*
  * Replaces adjointSRK (adjoint/srk_adjoint.cuh:402-425): for cotangent v
  * [W x W], dLdI = lambda_f [N x N] from the given ux, uy [W x W]
  * (adjoint/srk_adjoint.cuh:25-75) and dLdE = lambda_g [N x N], the true

@@ -37,6 +37,7 @@ SIGNATURES = {
     "srkTrainingForward": (_i, [_p, _p, _p, _p, _i, _i, _i, _f, _i, _p, _p, _p]),
     "srkCheckSpectraSupport": (_i, [_p, _p, _i, _i, _p, _p, ctypes.POINTER(_d)]),
     "srkForwardOaS": (_i, [_p, _p, _p, _p, _i, _i, _i, _f, _p, _p, _p, _p, _p]),
+    "srkForwardOaSNoAlloc": (_i, [_p, _p, _p, _p, _i, _i, _i, _f, _i, _p, _p, _p, _p]),
     "adjointSRK": (_i, [_p, _p, _p, _p, _p, _p, _i, _i, _i, _f, _p, _p, _p, _p]),
     "shrinkageTangentOaS": (_i, [_p, _p, _p, _p, _p, _p, _i, _i, _i, _f, _p, _p, _p]),
     "shrinkageTangentAdjointOaS": (_i, [_p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _f, _p,

@@ -14,7 +14,8 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libsrk_amd.so")
 
-SOURCES = ["srk_engine.hip", "srk_cd.hip", "srk_spectra.hip", "srk_api.cpp", "srk_patterns.cpp"]
+SOURCES = ["srk_engine.hip", "srk_cd.hip", "srk_spectra.hip", "srk_noalloc.hip", "srk_api.cpp",
+           "srk_patterns.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

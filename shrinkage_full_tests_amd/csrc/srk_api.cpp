@@ -214,6 +214,8 @@ static_assert(SRK_KERNEL_KINDS == srk::kKernelKinds, "kernel kinds of the C ABI 
 
 namespace srk {
 
+void note_hip_error(int e) { g_last_hip = e; }
+
 hipError_t LaunchTimer::record(hipStream_t s, hipEvent_t* ev)
 {
     if (used == pool.size()) {

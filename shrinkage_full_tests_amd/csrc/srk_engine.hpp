@@ -137,6 +137,9 @@ int engine_run(Op op, int M, int prec, const EngineArgs& args, int ntiles, int c
 int engine_import_spectra(int M, const float2* Gx_ref, const float2* Gy_ref, float scale,
                           float2* GT, hipStream_t stream);
 
+// Record a HIP error for srkLastHipError (the C ABI's per-thread last error).
+void note_hip_error(int hip_error);
+
 // lambda_f from caller-provided u (adjoint of the gradient stencil + combine).
 int engine_adjoint_f_from_u(const float* v, const float* ux, const float* uy, float* lam_f,
                             int N, int W, int e, int B, long long img_stride,
