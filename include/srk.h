@@ -153,7 +153,14 @@ int srkWorkspaceSetChunk(srk_workspace_t ws, int tiles_per_chunk); /* 0 = automa
 enum {
     SRK_TUNE_CHUNK_TILES = 0,  /* tiles per launch chunk (exchange buffers bound it)   */
     SRK_TUNE_P2_ROWS = 1,      /* k_y bands per row-pass block: 16, 8 or 4 (M <= 512) */
-    SRK_TUNE_CONCURRENCY = 2   /* 1 (default): chunks alternate between two streams; 0: one */
+    SRK_TUNE_CONCURRENCY = 2,  /* 1 (default): chunks alternate between two streams; 0: one */
+    SRK_TUNE_FUSED = 3,        /* 1: FP32 forward / adjoint of one-tile images (N <= M,
+                                  M = 256 or 512) run as one fused persistent launch;
+                                  0 (default): three passes per operator */
+    SRK_TUNE_FUSED_SLOTS = 4,  /* fused launch: exchange-ring slots per XCD, in
+                                  [lag + 1, 4] (default 4) */
+    SRK_TUNE_FUSED_LAG = 5     /* fused launch: rounds between the stages of one image,
+                                  in [1, slots - 1] (default 2) */
 };
 int srkWorkspaceSetTuning(srk_workspace_t ws, int key, int value);
 /* SRK_PREC_FP32 (default), SRK_PREC_MIXED_FP16 or SRK_PREC_BF16_EXCHANGE for every
@@ -171,11 +178,20 @@ int srkWorkspaceInfo(srk_workspace_t ws, int* M, int* fwd_tiles_per_image,
  * the summed duration in ms, the number of launches and the tiles they
  * processed (any array may be null), then forgets the records.
  */
-#define SRK_KERNEL_KINDS 6
+#define SRK_KERNEL_KINDS 7
 int srkWorkspaceSetTiming(srk_workspace_t ws, int enable);
 int srkWorkspaceKernelTimes(srk_workspace_t ws, double* total_ms, int* launches,
                             long long* tiles);
-const char* srkKernelKindName(int kind);  /* "k_p1_cols", ..., "k_p3_adj", or "" */
+const char* srkKernelKindName(int kind);  /* "k_p1_cols", ..., "k_p3_adj", "k_fused", or "" */
+
+/*
+ * Fused pipeline status (instrumentation; no counterpart in the reference):
+ * *available = 1 if the workspace can run the fused pipeline (N <= M,
+ * M = 256 or 512); *timeouts = waits of the last fused launch that passed
+ * their deadline (0 before the first fused launch; otherwise the launch's
+ * results are invalid, and SRK_ERR_HIP is returned).  Synchronises the device.
+ */
+int srkWorkspaceFusedStatus(srk_workspace_t ws, int* available, int* timeouts);
 
 int srkSynchronize(hipStream_t stream);
 

@@ -26,7 +26,7 @@ __all__ = [
 ]
 
 SUPPORTED_M = (256, 512, 1024, 2048)
-KERNEL_KINDS = 6  # SRK_KERNEL_KINDS
+KERNEL_KINDS = 7  # SRK_KERNEL_KINDS
 # SRK_PREC_FP32, SRK_PREC_MIXED_FP16, SRK_PREC_BF16_EXCHANGE
 PRECISIONS = {"fp32": 0, "mixed_fp16": 1, "bf16_exchange": 2}
 
@@ -117,15 +117,34 @@ class Workspace:
     def set_chunk(self, tiles: int) -> None:
         call("srkWorkspaceSetChunk", self.handle, int(tiles))
 
-    def set_tuning(self, chunk_tiles=None, p2_rows=None, concurrency=None) -> None:
+    def set_tuning(self, chunk_tiles=None, p2_rows=None, concurrency=None, fused=None,
+                   fused_slots=None, fused_lag=None) -> None:
         """Autotuner knobs: tiles per launch chunk, k_y bands per row-pass block,
-        chunks alternating between two streams (1, the default) or one (0)."""
+        chunks alternating between two streams (1, the default) or one (0), the
+        fused one-launch pipeline for one-tile images (1, or 0, the default), and
+        its exchange-ring slots per XCD and stage lag in rounds (lag < slots <= 4)."""
         if chunk_tiles is not None:
             call("srkWorkspaceSetTuning", self.handle, 0, int(chunk_tiles))
         if p2_rows is not None:
             call("srkWorkspaceSetTuning", self.handle, 1, int(p2_rows))
         if concurrency is not None:
             call("srkWorkspaceSetTuning", self.handle, 2, int(concurrency))
+        if fused is not None:
+            call("srkWorkspaceSetTuning", self.handle, 3, int(bool(fused)))
+        if fused_slots is not None and fused_lag is not None:
+            call("srkWorkspaceSetTuning", self.handle, 5, 1)  # below any slot count
+        if fused_slots is not None:
+            call("srkWorkspaceSetTuning", self.handle, 4, int(fused_slots))
+        if fused_lag is not None:
+            call("srkWorkspaceSetTuning", self.handle, 5, int(fused_lag))
+
+    def fused_status(self):
+        """(available, timed-out waits of the last fused launch); synchronises."""
+        av, to = ctypes.c_int(), ctypes.c_int()
+        st = lib().srkWorkspaceFusedStatus(self.handle, ctypes.byref(av), ctypes.byref(to))
+        if st not in (0, -8):
+            raise SrkError(st, "srkWorkspaceFusedStatus")
+        return bool(av.value), int(to.value)
 
     def set_precision(self, precision) -> None:
         """"fp32" (default), "mixed_fp16" (FP16 MFMA DFT stages, FP32 accumulation) or

@@ -32,6 +32,7 @@ SIGNATURES = {
     "srkWorkspaceSetTuning": (_i, [_p, _i, _i]),
     "srkWorkspaceInfo": (_i, [_p] + [ctypes.POINTER(_i)] * 5),
     "srkWorkspaceSetTiming": (_i, [_p, _i]),
+    "srkWorkspaceFusedStatus": (_i, [_p, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "srkWorkspaceKernelTimes": (_i, [_p, _p, _p, _p]),
     "srkKernelKindName": (ctypes.c_char_p, [_i]),
     "srkTrainingForward": (_i, [_p, _p, _p, _p, _i, _i, _i, _f, _i, _p, _p, _p]),

@@ -274,6 +274,13 @@ struct srk_workspace_s {
     void* rs_scratch = nullptr;
     float2* rs_g = nullptr;  // [2][M][M/2+1], reference layout
     int rs_ns_max = 0;
+    // fused pipeline (N <= M, M = 256 or 512): exchange rings of 16 slots and
+    // the queue counters; used for FP32 forward / adjoint calls when enabled
+    float2* X1f = nullptr;
+    float2* X2f = nullptr;
+    void* fsync = nullptr;
+    bool fused = false;
+    int fused_slots = 4, fused_lag = 2;  // ring slots per XCD, stage lag in rounds
     // completion of the last call (recorded on its stream): a call on another
     // stream waits for it, since calls share the exchange buffers and spectra
     hipEvent_t done = nullptr;
@@ -299,6 +306,9 @@ void ws_free(srk_workspace_s* w)
     hipFree(w->tw);
     hipFree(w->rs_scratch);
     hipFree(w->rs_g);
+    hipFree(w->X1f);
+    hipFree(w->X2f);
+    hipFree(w->fsync);
     delete w;
 }
 
@@ -459,6 +469,42 @@ const srk::Fork* lanes_for(srk_workspace_s* w, const srk::EngineArgs& a)
     return &w->fork;
 }
 
+// Exchange rings of the fused pipeline, allocated when it is first enabled:
+// kFusedSlotsMax slots per XCD of up to 3 X1 planes (adjoint: g, a_x, a_y)
+// and 3 X2 planes of max(W, N + 2) columns, and the queue counters.
+int fused_buffers(srk_workspace_s* w)
+{
+    if (w->X1f || !srk::fused_supported(srk::Op::Forward, w->M, w->N, 0)) return SRK_OK;
+    const int M = w->M, HP = srk::spec_pitch(M);
+    const size_t slots = (size_t)8 * srk::kFusedSlotsMax;
+    const size_t b1 = slots * 3 * M * HP * sizeof(float2);
+    const size_t b2 = slots * 3 * std::max(w->S2f, w->S2a) * HP * sizeof(float2);
+    const size_t bs = srk::fused_sync_bytes();
+    float2 *x1 = nullptr, *x2 = nullptr;
+    void* sy = nullptr;
+    hipError_t e = hipMalloc((void**)&x1, b1);
+    if (e == hipSuccess) e = hipMalloc((void**)&x2, b2);
+    if (e == hipSuccess) e = hipMalloc(&sy, bs);
+    if (e == hipSuccess) e = hipMemset(sy, 0, bs);
+    if (e != hipSuccess) {
+        hipFree(x1);
+        hipFree(x2);
+        hipFree(sy);
+        return hip_status(e);
+    }
+    w->X1f = x1;
+    w->X2f = x2;
+    w->fsync = sy;
+    w->total += b1 + b2 + bs;
+    return SRK_OK;
+}
+
+// the fused pipeline serves this call
+bool use_fused(const srk_workspace_s* w, srk::Op op)
+{
+    return w->fused && w->X1f && srk::fused_supported(op, w->M, w->N, w->precision);
+}
+
 int run_fwd(srk_workspace_s* w, srk::Op op, srk::EngineArgs a, int B, hipStream_t s)
 {
     const int nf = op == srk::Op::Tangent ? 2 : 1;
@@ -467,6 +513,13 @@ int run_fwd(srk_workspace_s* w, srk::Op op, srk::EngineArgs a, int B, hipStream_
     a.np2 = 2 * nf;
     a.S2 = w->S2f;
     a.halo2 = 0;
+    if (use_fused(w, op)) {
+        a.X1 = w->X1f;
+        a.X2 = w->X2f;
+        SRK_TRY(hip_status(srk::engine_run_fused(op, w->M, a, B, w->fsync, w->fused_slots,
+                                                 w->fused_lag, s, timer_for(w, s))));
+        return ws_record(w, s);
+    }
     const int ntiles = B * w->nft;
     const int chunk = chunk_for(w, a.np1, a.np2, a.S2, ntiles);
     SRK_TRY(hip_status(srk::engine_run(op, w->M, w->precision, a, ntiles, chunk, s,
@@ -485,6 +538,13 @@ int run_adj(srk_workspace_s* w, srk::Op op, srk::EngineArgs a, int B, hipStream_
     a.tiles = w->d_atiles;
     a.S2 = w->S2a;
     a.halo2 = 1;
+    if (use_fused(w, op)) {
+        a.X1 = w->X1f;
+        a.X2 = w->X2f;
+        SRK_TRY(hip_status(srk::engine_run_fused(op, w->M, a, B, w->fsync, w->fused_slots,
+                                                 w->fused_lag, s, timer_for(w, s))));
+        return ws_record(w, s);
+    }
     const int ntiles = B * w->nat;
     const int chunk = chunk_for(w, a.np1, a.np2, a.S2, ntiles);
     SRK_TRY(hip_status(srk::engine_run(op, w->M, w->precision, a, ntiles, chunk, s,
@@ -673,6 +733,13 @@ int srkWorkspaceCreate(int N, int W, int erosion, int M, int B_max, size_t scrat
     }
     if (const char* env = std::getenv("SRK_CHUNK_TILES")) w->chunk_override = std::atoi(env);
     if (const char* env = std::getenv("SRK_P2_ROWS")) w->p2_rows = std::atoi(env);
+    if (const char* env = std::getenv("SRK_FUSED")) w->fused = std::atoi(env) != 0;
+    if (w->fused && (st = fused_buffers(w)) != SRK_OK) {
+        ws_free(w);
+        return st;
+    }
+    if (const char* env = std::getenv("SRK_FUSED_SLOTS")) w->fused_slots = std::atoi(env);
+    if (const char* env = std::getenv("SRK_FUSED_LAG")) w->fused_lag = std::atoi(env);
     *out = w;
     return SRK_OK;
 }
@@ -710,9 +777,35 @@ int srkWorkspaceSetTuning(srk_workspace_t w, int key, int value)
         if (value != 0 && value != 1) return SRK_ERR_INVALID_ARG;
         w->concurrent = value != 0;
         return SRK_OK;
+    case SRK_TUNE_FUSED:
+        if (value != 0 && value != 1) return SRK_ERR_INVALID_ARG;
+        if (value) SRK_TRY(fused_buffers(w));
+        w->fused = value != 0;
+        return SRK_OK;
+    case SRK_TUNE_FUSED_SLOTS:
+        if (value <= w->fused_lag || value > srk::kFusedSlotsMax) return SRK_ERR_INVALID_ARG;
+        w->fused_slots = value;
+        return SRK_OK;
+    case SRK_TUNE_FUSED_LAG:
+        if (value < 1 || value >= w->fused_slots) return SRK_ERR_INVALID_ARG;
+        w->fused_lag = value;
+        return SRK_OK;
     default:
         return SRK_ERR_INVALID_ARG;
     }
+}
+
+int srkWorkspaceFusedStatus(srk_workspace_t w, int* available, int* timeouts)
+{
+    if (!w) return SRK_ERR_NULL_PTR;
+    if (available) *available = srk::fused_supported(srk::Op::Forward, w->M, w->N, 0) ? 1 : 0;
+    if (timeouts) *timeouts = 0;
+    if (!w->fsync) return SRK_OK;
+    int t = 0;
+    SRK_TRY(hip_status(hipDeviceSynchronize()));
+    SRK_TRY(hip_status(srk::fused_sync_status(w->fsync, &t)));
+    if (timeouts) *timeouts = t;
+    return t == 0 ? SRK_OK : SRK_ERR_HIP;
 }
 
 int srkWorkspaceSetTiming(srk_workspace_t w, int enable)
@@ -736,7 +829,8 @@ int srkWorkspaceKernelTimes(srk_workspace_t w, double* total_ms, int* launches, 
 const char* srkKernelKindName(int kind)
 {
     static const char* names[SRK_KERNEL_KINDS] = {"k_p1_cols", "k_p1_adj_src", "k_p2_conv",
-                                                  "k_p2_corr", "k_p3_fwd",    "k_p3_adj"};
+                                                  "k_p2_corr", "k_p3_fwd",    "k_p3_adj",
+                                                  "k_fused"};
     return kind >= 0 && kind < SRK_KERNEL_KINDS ? names[kind] : "";
 }
 

@@ -42,6 +42,31 @@ __device__ __forceinline__ void stu(T* base, unsigned idx, T v)
     *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + idx * (unsigned)sizeof(T)) = v;
 }
 
+// Load of an exchange-buffer element at base[idx].  COH: an agent-scope
+// relaxed atomic load, which misses the CU's own L1 and is served by the
+// XCD's L2 -- for elements another workgroup of the same launch wrote (the
+// fused pipeline publishes them with store completion and a flag).
+template <bool COH, class T>
+__device__ __forceinline__ T ldc(const T* base, unsigned idx)
+{
+    if constexpr (!COH) {
+        return ldu(base, idx);
+    } else {
+        const char* p = reinterpret_cast<const char*>(base) + idx * (unsigned)sizeof(T);
+        if constexpr (sizeof(T) == 8) {
+            const unsigned long long u = __hip_atomic_load(
+                reinterpret_cast<unsigned long long*>(const_cast<char*>(p)), __ATOMIC_RELAXED,
+                __HIP_MEMORY_SCOPE_AGENT);
+            return __builtin_bit_cast(T, u);
+        } else {
+            static_assert(sizeof(T) == 4, "exchange elements are 4 or 8 bytes");
+            const unsigned u = __hip_atomic_load(reinterpret_cast<unsigned*>(const_cast<char*>(p)),
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return __builtin_bit_cast(T, u);
+        }
+    }
+}
+
 // Element formats of the exchange planes: FP32 complex (float2, 8 bytes) or
 // a bfloat16 pair (Bf16c, 4 bytes, real part in the low half), rounded to
 // nearest even on store (v_cvt_pk_bf16_f32).  A plane keeps its FP32-sized
@@ -82,7 +107,7 @@ __device__ __forceinline__ S* plane_at(float2* X, long long index, long long ele
 // for the lane's transform inputs k = lane + 64 i (raw, so that a load can be
 // issued one transform ahead), and their packing Z = P + iQ.  The DC and
 // Nyquist bins are taken as real, as a real inverse transform does.
-template <int M, class S>
+template <int M, class S, bool COH = false>
 struct PairRaw {
     static constexpr int V = M / kWave;
     S p[V], q[V];
@@ -93,8 +118,8 @@ struct PairRaw {
         for (int i = 0; i < V; ++i) {
             const int k = lane + kWave * i;
             const unsigned kk = k <= M / 2 ? k : M - k;
-            p[i] = ldu(P, kk);
-            q[i] = ldu(Q, kk);
+            p[i] = ldc<COH>(P, kk);
+            q[i] = ldc<COH>(Q, kk);
         }
     }
     __device__ __forceinline__ void pack(float2 (&v)[V], int lane) const
@@ -115,11 +140,11 @@ struct PairRaw {
 
 // Inverse transform of P + iQ; afterwards out[i] = p(y) + i q(y) at tile row
 // y = lane + 64 i.
-template <int M, class PL, class S>
+template <int M, class PL, class S, bool COH = false>
 __device__ __forceinline__ void ifft_pair(const PL& FP, float2* line, const S* P, const S* Q,
                                           float2 (&out)[M / kWave], int lane)
 {
-    PairRaw<M, S> r;
+    PairRaw<M, S, COH> r;
     r.load(P, Q, lane);
     float2 v[M / kWave];
     r.pack(v, lane);
@@ -148,22 +173,22 @@ __device__ __forceinline__ float2 grad_in(const Col3& f, unsigned yy)
 // P1: real FFT along y of the convolved fields; one wavefront per column pair,
 // 8 columns per block.
 // ---------------------------------------------------------------------------
+// Column block bx (8 columns) of tile t; exchange slot `slot` of X1; LDS
+// lines lds[4].  256 threads.
 template <int M, class PL, int NF, class S>
-__global__ __launch_bounds__(256) void k_p1_cols(EngineArgs a)
+__device__ __forceinline__ void p1_cols_item(const EngineArgs& a, const Tile& t, int slot, int bx,
+                                             float2 (*lds)[line_slots(M)])
 {
     constexpr int nf = NF;
     constexpr int V = M / kWave;
     constexpr int H = M / 2 + 1;
     constexpr int HP = spec_pitch(M);
     constexpr int KV = (H + kWave - 1) / kWave;  // half-spectrum values per lane
-    __shared__ float2 lds[4][line_slots(M)];
     const int lane = lane_id(), w = wave_id();
     PL FP;
     FP.load(a.tw, lane);
-    const int slot = blockIdx.y;
-    const Tile t = a.tiles[slot];
     const int N = a.N;
-    const int x0 = blockIdx.x * 8;
+    const int x0 = bx * 8;
     const int xa = x0 + 2 * w;
     const int ca = t.p0x + xa, cb = ca + 1;
     const bool va = ca >= 0 && ca < N, vb = cb >= 0 && cb < N;
@@ -234,6 +259,13 @@ __global__ __launch_bounds__(256) void k_p1_cols(EngineArgs a)
     }
 }
 
+template <int M, class PL, int NF, class S>
+__global__ __launch_bounds__(256) void k_p1_cols(EngineArgs a)
+{
+    __shared__ float2 lds[4][line_slots(M)];
+    p1_cols_item<M, PL, NF, S>(a, a.tiles[blockIdx.y], blockIdx.y, blockIdx.x, lds);
+}
+
 // ---------------------------------------------------------------------------
 // P1A: adjoint source a_d = v0 * d_d f0 (+ v1 * d_d f1) on the valid region,
 // transformed along y as a_x + i a_y; one wavefront per column, 4 per block,
@@ -242,22 +274,21 @@ __global__ __launch_bounds__(256) void k_p1_cols(EngineArgs a)
 // loads need no bounds checks: rows outside it load at a clamped row and are
 // masked.
 // ---------------------------------------------------------------------------
+// Column block bx (4 columns) of tile t; 256 threads.
 template <int M, class PL, int NF, class S>
-__global__ __launch_bounds__(256) void k_p1_adj_src(EngineArgs a, int plane0)
+__device__ __forceinline__ void p1_adj_src_item(const EngineArgs& a, int plane0, const Tile& t,
+                                                int slot, int bx, float2 (*lds)[line_slots(M)])
 {
     constexpr int nf = NF;
     constexpr int V = M / kWave;
     constexpr int HP = spec_pitch(M);
-    __shared__ float2 lds[4][line_slots(M)];
     const int lane = lane_id(), w = wave_id();
-    const int slot = blockIdx.y;
-    const Tile t = a.tiles[slot];
     const int N = a.N, e = a.e, W = a.W;
     // a block whose four columns all lie off the valid region has only zero
     // sources; P2 corr does not read those columns
-    const int gb = t.p0x + blockIdx.x * 4;
+    const int gb = t.p0x + bx * 4;
     if (gb + 3 < e || gb >= N - e) return;
-    const int x = blockIdx.x * 4 + w;
+    const int x = bx * 4 + w;
     const int gx = t.p0x + x;
     constexpr int H = M / 2 + 1;
     constexpr int KV = (H + kWave - 1) / kWave;
@@ -315,7 +346,7 @@ __global__ __launch_bounds__(256) void k_p1_adj_src(EngineArgs a, int plane0)
     __syncthreads();
     // block-wide store of columns x0 .. x0+3 of both planes into the band layout:
     // element e = (band b, column c, row r), e = b 4RY + c RY + r
-    const int x0 = blockIdx.x * 4;
+    const int x0 = bx * 4;
     const int RY = 1 << a.ry_log2;
 #pragma unroll
     for (int pl = 0; pl < 2; ++pl) {
@@ -329,6 +360,13 @@ __global__ __launch_bounds__(256) void k_p1_adj_src(EngineArgs a, int plane0)
             stx(plane, x1_index<M>(M / 2, x0 + threadIdx.x, a.ry_log2),
                 lds[threadIdx.x][pl * H + M / 2]);
     }
+}
+
+template <int M, class PL, int NF, class S>
+__global__ __launch_bounds__(256) void k_p1_adj_src(EngineArgs a, int plane0)
+{
+    __shared__ float2 lds[4][line_slots(M)];
+    p1_adj_src_item<M, PL, NF, S>(a, plane0, a.tiles[blockIdx.y], blockIdx.y, blockIdx.x, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -405,7 +443,9 @@ struct BandRegs {
     static_assert((T / RY) % 8 == 0 && (M * RY) % T == 0, "band load shape");
     S v[IT];  // as stored: a bfloat16 prefetch holds half the registers
 
-    // columns outside [xlo, xhi) (tile-local) are taken as zero and not read
+    // columns outside [xlo, xhi) (tile-local) are taken as zero and not read;
+    // COH: the band was written by another workgroup of the same launch
+    template <bool COH = false>
     __device__ __forceinline__ void fetch(const S* __restrict__ band, int nrows, int xlo = 0,
                                           int xhi = M)
     {
@@ -417,7 +457,7 @@ struct BandRegs {
         for (int i = 0; i < IT; ++i) {
             const int x = x0 + i * XS;
             v[i] = pack<S>(make_float2(0.f, 0.f));
-            if (ok && x >= xlo && x < xhi) v[i] = ldu(src, i * T);
+            if (ok && x >= xlo && x < xhi) v[i] = ldc<COH>(src, i * T);
         }
     }
     __device__ __forceinline__ void put(float2* L) const
@@ -706,19 +746,150 @@ void k_p2_corr(EngineArgs a, int planeIn, int planeOut, int tiles)
 }
 
 // ---------------------------------------------------------------------------
+// Row passes of the fused pipeline (k_fused below): one band of kFusedRY k_y
+// rows of one tile per call, kFusedNW wavefronts with kFusedRY / kFusedNW rows
+// each and one LDS line per row (kFusedRY lines of pitch p2_pitch, 37.8 KB at
+// M = 512), so that four 256-thread workgroups share a CU.  The per-row
+// arithmetic is that of k_p2_conv / k_p2_corr; X1 bands are read with
+// coherent loads (written by other workgroups of the launch).
+// ---------------------------------------------------------------------------
+constexpr int kFusedRY = 8, kFusedNW = 4;
+
+template <int M>
+constexpr int fused_p2_floats() { return kFusedRY * p2_pitch<M, kFusedRY>() * 2; }
+
+// u_x, u_y of band `band` of field 0 (X1 plane 0 -> X2 planes 0, 1)
+template <int M, class PL, class S>
+__device__ __forceinline__ void p2_conv_band_fused(const EngineArgs& a, const Tile& t, int slot,
+                                                   int band, float2* A)
+{
+    constexpr int RY = kFusedRY, NW = kFusedNW, RPW = RY / NW;
+    constexpr int V = M / kWave, H = M / 2 + 1, HP = spec_pitch(M), LP = p2_pitch<M, RY>();
+    constexpr int NBF = M / (2 * RY);
+    const int lane = lane_id(), w = wave_id();
+    PL FP;
+    FP.load(a.tw, lane);
+    const int k0 = band * RY, nrows = band < NBF ? RY : 1;
+    {
+        BandRegs<M, RY, NW, S> pf;
+        pf.template fetch<true>(
+            plane_at<const S>(a.X1, (long long)slot * a.np1, (long long)M * HP) +
+                (long long)band * RY * M,
+            nrows);
+        pf.put(A);
+    }
+    __syncthreads();
+    float2 U[RPW][V];
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+        const int r = w + q * NW;
+        if (r < nrows) {
+            float2* la = A + r * LP;
+            const float2* gxr = a.GT + (long long)(k0 + r) * M + lane;
+            const float2* gyr = a.GT + (long long)(H + k0 + r) * M + lane;
+#pragma unroll
+            for (int i = 0; i < V; ++i) U[q][i] = ldu(gyr, kWave * i);
+            float2 X[V];
+            fft_lds_to_regs<M, -1>(FP, la, X, lane);
+#pragma unroll
+            for (int i = 0; i < V; ++i) {
+                U[q][i] = cmul(X[i], U[q][i]);
+                X[i] = cmul(X[i], ldu(gxr, kWave * i));
+            }
+            fft_from_regs<M, +1>(FP, la, X, lane);
+        }
+    }
+    __syncthreads();
+    const int lo = t.ox0 - t.p0x - a.halo2;
+    int xs, xe;
+    u_columns(a, t, xs, xe);
+    const long long pe = (long long)a.S2 * HP;
+    store_band<M, RY, NW>(plane_at<S>(a.X2, (long long)slot * a.np2, pe), A, k0, nrows, lo, xs, xe);
+    __syncthreads();  // every row of u_x is stored: the lines take u_y
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+        const int r = w + q * NW;
+        if (r < nrows) fft_from_regs<M, +1>(FP, A + r * LP, U[q], lane);
+    }
+    __syncthreads();
+    store_band<M, RY, NW>(plane_at<S>(a.X2, (long long)slot * a.np2 + 1, pe), A, k0, nrows, lo, xs,
+                          xe);
+}
+
+// conj(Gx) A_x + conj(Gy) A_y of band `band` (X1 planes planeIn, planeIn + 1
+// -> X2 plane planeOut)
+template <int M, class PL, class S>
+__device__ __forceinline__ void p2_corr_band_fused(const EngineArgs& a, const Tile& t, int slot,
+                                                   int band, int planeIn, int planeOut, float2* A)
+{
+    constexpr int RY = kFusedRY, NW = kFusedNW, RPW = RY / NW;
+    constexpr int V = M / kWave, H = M / 2 + 1, HP = spec_pitch(M), LP = p2_pitch<M, RY>();
+    constexpr int NBF = M / (2 * RY);
+    const int lane = lane_id(), w = wave_id();
+    PL FP;
+    FP.load(a.tw, lane);
+    const int k0 = band * RY, nrows = band < NBF ? RY : 1;
+    auto band_src = [&](int p) {
+        return plane_at<const S>(a.X1, (long long)slot * a.np1 + planeIn + p, (long long)M * HP) +
+               (long long)band * RY * M;
+    };
+    // the sources vanish off the valid region: P1A leaves columns there unwritten
+    const int sx0 = a.e - t.p0x, sx1 = a.N - a.e - t.p0x;
+    BandRegs<M, RY, NW, S> pb;
+    {
+        BandRegs<M, RY, NW, S> pa;
+        pa.template fetch<true>(band_src(0), nrows, sx0, sx1);
+        pa.put(A);
+    }
+    __syncthreads();
+    pb.template fetch<true>(band_src(1), nrows, sx0, sx1);  // in flight during the transforms
+    float2 X[RPW][V];
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+        const int r = w + q * NW;
+        if (r < nrows) fft_lds_to_regs<M, -1>(FP, A + r * LP, X[q], lane);
+    }
+    __syncthreads();  // every line is read: the lines take the second plane
+    pb.put(A);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+        const int r = w + q * NW;
+        if (r < nrows) {
+            const float2* gxr = a.GT + (long long)(k0 + r) * M + lane;
+            const float2* gyr = a.GT + (long long)(H + k0 + r) * M + lane;
+            float2 G[V], Y[V];
+#pragma unroll
+            for (int i = 0; i < V; ++i) G[i] = ldu(gyr, kWave * i);
+            fft_lds_to_regs<M, -1>(FP, A + r * LP, Y, lane);
+#pragma unroll
+            for (int i = 0; i < V; ++i)
+                X[q][i] = cadd(cmulc(X[q][i], ldu(gxr, kWave * i)), cmulc(Y[i], G[i]));
+            fft_from_regs<M, +1>(FP, A + r * LP, X[q], lane);
+        }
+    }
+    __syncthreads();
+    const int lo = t.ox0 - t.p0x - a.halo2;
+    const int cnt = t.ox1 - t.ox0 + 2 * a.halo2;
+    store_band<M, RY, NW>(plane_at<S>(a.X2, (long long)slot * a.np2 + planeOut,
+                                      (long long)a.S2 * HP),
+                          A, k0, nrows, lo, 0, cnt);
+}
+
+// ---------------------------------------------------------------------------
 // P3 forward / tangent: inverse y-FFT of (u_x + i u_y) per output column,
 // gradient stencil and combine, valid-region store.
 // ---------------------------------------------------------------------------
-template <int M, class PL, int NF, class S>
-__global__ __launch_bounds__(256) void k_p3_fwd(EngineArgs a)
+// Output columns 4 bx .. 4 bx + 3 of tile t (one per wavefront); 256 threads.
+// COH: the X2 columns were written by another workgroup of the same launch.
+template <int M, class PL, int NF, class S, bool COH = false>
+__device__ __forceinline__ void p3_fwd_item(const EngineArgs& a, const Tile& t, int slot, int bx,
+                                            float2 (*lds)[line_slots(M)])
 {
     constexpr int V = M / kWave;
     constexpr int HP = spec_pitch(M);
-    __shared__ float2 lds[4][line_slots(M)];
     const int lane = lane_id(), w = wave_id();
-    const int slot = blockIdx.y;
-    const Tile t = a.tiles[slot];
-    const int xi = blockIdx.x * 4 + w;
+    const int xi = bx * 4 + w;
     if (xi >= t.ox1 - t.ox0) return;
     PL FP;
     FP.load(a.tw, lane);
@@ -738,7 +909,7 @@ __global__ __launch_bounds__(256) void k_p3_fwd(EngineArgs a)
 
     // u(g) for this column at tile rows y = lane + 64 j
     float2 u0[V];
-    ifft_pair<M, PL>(FP, line, col(0), col(1), u0, lane);
+    ifft_pair<M, PL, S, COH>(FP, line, col(0), col(1), u0, lane);
     if constexpr (NF == 1) {
 #pragma unroll
         for (int j = 0; j < V; ++j) {
@@ -762,7 +933,7 @@ __global__ __launch_bounds__(256) void k_p3_fwd(EngineArgs a)
         // tangent: + u(dg) . grad f + u(g) . grad df
         const Col3 F1 = col3(a.f1 + (long long)t.b * a.img_stride, N, gx);
         float2 u1[V];
-        ifft_pair<M, PL>(FP, line, col(2), col(3), u1, lane);
+        ifft_pair<M, PL, S, COH>(FP, line, col(2), col(3), u1, lane);
 #pragma unroll
         for (int j = 0; j < V; ++j) {
             const int gy = t.p0y + lane + kWave * j;
@@ -773,6 +944,13 @@ __global__ __launch_bounds__(256) void k_p3_fwd(EngineArgs a)
             }
         }
     }
+}
+
+template <int M, class PL, int NF, class S>
+__global__ __launch_bounds__(256) void k_p3_fwd(EngineArgs a)
+{
+    __shared__ float2 lds[4][line_slots(M)];
+    p3_fwd_item<M, PL, NF, S>(a, a.tiles[blockIdx.y], blockIdx.y, blockIdx.x, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -790,20 +968,23 @@ __global__ __launch_bounds__(256) void k_p3_fwd(EngineArgs a)
 // ---------------------------------------------------------------------------
 constexpr int P3A_WAVES = 4;
 
-template <int M, class PL, int NF, class SL>
-__global__ __launch_bounds__(P3A_WAVES * kWave)
-__attribute__((amdgpu_waves_per_eu(M <= 512 && NF == 1 && !IsMfmaPlan<PL>::value ? 4 : 1))) void k_p3_adj(EngineArgs a, int with_u, int planeL)
+// w_x of the 2 P3A_WAVES + 2 columns of a P3 adjoint block, rows y
+template <int M>
+constexpr int p3a_wx_floats() { return (2 * P3A_WAVES + 2) * M; }
+
+// Output columns CBK bx .. CBK bx + CBK - 1 of tile t; P3A_WAVES wavefronts.
+// lds: one line per wavefront; wx: p3a_wx_floats<M>() floats.  COH as p3_fwd_item.
+template <int M, class PL, int NF, class SL, bool COH = false>
+__device__ __forceinline__ void p3_adj_item(const EngineArgs& a, int with_u, int planeL,
+                                            const Tile& t, int slot, int bx,
+                                            float2 (*lds)[line_slots(M)], float* wx)
 {
     constexpr int nf = NF;
     constexpr int V = M / kWave;
     constexpr int HP = spec_pitch(M);
     constexpr int CBK = 2 * P3A_WAVES;  // output columns per block
-    __shared__ float2 lds[P3A_WAVES][line_slots(M)];
-    __shared__ float wx[(CBK + 2) * M];  // w_x of block columns -1 .. CBK, rows y
     const int lane = lane_id(), w = wave_id();
-    const int slot = blockIdx.y;
-    const Tile t = a.tiles[slot];
-    const int jb = blockIdx.x * CBK;
+    const int jb = bx * CBK;
     const int cnt = t.ox1 - t.ox0;
     if (jb >= cnt) return;  // whole block beyond the tile's columns
     const int j0 = jb + 2 * w;
@@ -857,7 +1038,7 @@ __attribute__((amdgpu_waves_per_eu(M <= 512 && NF == 1 && !IsMfmaPlan<PL>::value
             auto u_pair = [&](int j) {
                 const int c = t.ox0 + j;
                 if (c >= e && c < N - e) {
-                    ifft_pair<M, PL>(FP, line, col(px, j), col(py, j), u, lane);
+                    ifft_pair<M, PL, float2, COH>(FP, line, col(px, j), col(py, j), u, lane);
                 } else {
 #pragma unroll
                     for (int q = 0; q < V; ++q) u[q] = make_float2(0.f, 0.f);
@@ -894,7 +1075,8 @@ __attribute__((amdgpu_waves_per_eu(M <= 512 && NF == 1 && !IsMfmaPlan<PL>::value
                 cot(cl, cv);
                 cot(cr0, cr);
                 if ((cl >= e && cl < N - e) || (cr0 >= e && cr0 < N - e)) {
-                    ifft_pair<M, PL>(FP, line, col(px, jb - 1), col(px, jb + CBK), u, lane);
+                    ifft_pair<M, PL, float2, COH>(FP, line, col(px, jb - 1), col(px, jb + CBK), u,
+                                                  lane);
                 } else {
 #pragma unroll
                     for (int q = 0; q < V; ++q) u[q] = make_float2(0.f, 0.f);
@@ -931,7 +1113,8 @@ __attribute__((amdgpu_waves_per_eu(M <= 512 && NF == 1 && !IsMfmaPlan<PL>::value
     if (j0 < cnt) {
         const bool second = j0 + 1 < cnt;
         float2 l[V];
-        ifft_pair<M, PL>(FP, line, colS(SL{}, planeL, j0), colS(SL{}, planeL, j0 + 1), l, lane);
+        ifft_pair<M, PL, SL, COH>(FP, line, colS(SL{}, planeL, j0), colS(SL{}, planeL, j0 + 1), l,
+                                  lane);
         float* o1 = a.out1 + ib + (long long)gx * N;
 #pragma unroll
         for (int j = 0; j < V; ++j) {
@@ -942,6 +1125,16 @@ __attribute__((amdgpu_waves_per_eu(M <= 512 && NF == 1 && !IsMfmaPlan<PL>::value
             }
         }
     }
+}
+
+template <int M, class PL, int NF, class SL>
+__global__ __launch_bounds__(P3A_WAVES * kWave)
+__attribute__((amdgpu_waves_per_eu(M <= 512 && NF == 1 && !IsMfmaPlan<PL>::value ? 4 : 1))) void k_p3_adj(EngineArgs a, int with_u, int planeL)
+{
+    __shared__ float2 lds[P3A_WAVES][line_slots(M)];
+    __shared__ float wx[p3a_wx_floats<M>()];
+    p3_adj_item<M, PL, NF, SL>(a, with_u, planeL, a.tiles[blockIdx.y], blockIdx.y, blockIdx.x,
+                               lds, wx);
 }
 
 // lambda_f from caller-provided u_x, u_y [W x W] (adjoint/srk_adjoint.cuh:25-59).
@@ -989,6 +1182,287 @@ __global__ void k_import_spectra(const float2* __restrict__ Gx, const float2* __
         if (kx < M && ky < H) {
             const float2 g = tile[threadIdx.x][i];
             dst[(long long)ky * M + kx] = make_float2(g.x * scale, g.y * scale);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused pipeline: the three passes of Op::Forward or Op::Adjoint for batches
+// of one-tile images (N <= M), in one persistent launch.  Each workgroup reads
+// the hardware id of its XCD and serves that XCD's queue: the XCD owns images
+// b = 8 j + xcd (j = 0, 1, ...), and its exchange planes live in a ring of K
+// slots per XCD (image j in slot j mod K), so an image's planes are written
+// and read through that XCD's L2 and the Infinity Cache instead of
+// round-tripping HBM.
+//
+// Queue order: round r holds the stage-3 items (column inverse transforms)
+// of image r - 2L, then the stage-2 items (row passes) of image r - L, then
+// the stage-1 items (column transforms) of image r.  Stage 1 of image j waits
+// for stage 2 of image j - K (its X1 slot), stage 2 of j for stage 1 of j and
+// for stage 3 of j - K (its X2 slot), stage 3 of j for stage 2 of j.  With
+// K > L every item depends only on items of earlier rounds, so items claimed
+// in order can never wait on an item no running workgroup holds; with
+// K >= L + 2 and L >= 2 the items waited on were claimed two or more rounds
+// earlier, so in steady state they are complete and no round waits on the
+// stragglers of the previous one.  A stage publishes its planes by waiting
+// for its stores to complete (they are then in the XCD's L2), a barrier, and
+// one relaxed agent-scope increment of the slot's counter; a consumer polls
+// the counter (relaxed agent-scope loads) and reads the planes with coherent
+// loads that miss its CU's L1.  Every wait is bounded by a deadline: past it
+// the workgroup raises the abort flag and counts a timeout
+// (srk::fused_sync_status), and all workgroups drain the queue without work.
+//
+// Every value that steers the loop (claimed item, wait results) is made
+// wavefront-uniform (readfirstlane) and the loop has a single latch, and the
+// claim and the polls run in wave-uniform branches.  When the item index was
+// read from LDS as a per-lane value and the claim, the polls and the
+// deadline check were done by thread 0 alone, the compiler built the loop as
+// a per-lane loop in which thread 0 left to claim the next item while the
+// other lanes re-ran the last one without end (the workgroup never reached a
+// barrier it needed, and the deadline check was on thread 0's path only).
+// ---------------------------------------------------------------------------
+#ifndef FUSED_WPE
+#define FUSED_WPE 4
+#endif
+constexpr int kSyncLine = 32;  // ints per 128-byte line
+
+struct FusedSync {
+    int abort[kSyncLine];
+    int status[kSyncLine];                      // [0] timed-out waits
+    int x[8][1 + 3 * kFusedSlotsMax][kSyncLine];  // per XCD: claim counter, c1[K], c2[K], c3[K]
+};
+
+struct FusedArgs {
+    EngineArgs a;                // X1 / X2: rings of 8 K slots (slot = K xcd + j mod K)
+    FusedSync* sync;
+    int K, L;                    // ring slots per XCD, stage lag in rounds
+    int B;                       // images
+    int n1a, n1b;                // stage 1: column blocks of P1 (n1a), then of P1A (n1b)
+    int bx1b;                    // first column block of the P1A part
+    int n2;                      // stage 2: bands per row pass (NBF + 1)
+    int n3;                      // stage 3: column blocks of P3
+    unsigned long long budget;   // wall-clock ticks (100 MHz) a workgroup may run
+};
+
+__device__ __forceinline__ int xcd_id()
+{
+    return __builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 7;  // HW_REG_XCC_ID[3:0]
+}
+
+__device__ __forceinline__ int sync_load(int* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ long long uni(long long v)
+{
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)((unsigned long long)v >> 32));
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+// index of the calling wavefront, uniform in the compiler's view
+__device__ __forceinline__ int wave_index() { return uni((int)(threadIdx.x >> 6)); }
+
+// Wait until *ctr >= target; false after the deadline or an abort.  Wavefront 0
+// polls with uniform control (every lane loads the counter, the value is made
+// uniform), so the poll loop is a scalar loop; the other wavefronts wait at
+// the barrier and learn the result through LDS.
+__device__ bool fused_wait(FusedSync* sy, int* ctr, int target, unsigned long long t_end,
+                           int* flag)
+{
+    if (wave_index() == 0) {
+        int ok = 1;
+        for (;;) {
+            if (uni(sync_load(ctr)) >= target) break;
+            if (uni(sync_load(&sy->abort[0])) != 0) {
+                ok = 0;
+                break;
+            }
+            if (wall_clock64() > t_end) {
+                if (threadIdx.x == 0) {
+                    __hip_atomic_fetch_add(&sy->status[0], 1, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(&sy->abort[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        *flag = ok;  // every lane of wavefront 0 writes the same value
+    }
+    __syncthreads();
+    const int ok = uni(*flag);
+    __syncthreads();  // every wavefront has read the flag before it is rewritten
+    return ok != 0;
+}
+
+// publish this block's stores: complete them (into the L2), then count the item
+__device__ __forceinline__ void fused_signal(int* ctr)
+{
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int M>
+constexpr int fused_lds_bytes()
+{
+    constexpr int lines = 4 * line_slots(M) * 8;                        // P1, P1A, P3 fwd
+    constexpr int p2 = kFusedRY * p2_pitch<M, kFusedRY>() * 8;           // row passes
+    constexpr int p3a = P3A_WAVES * line_slots(M) * 8 + p3a_wx_floats<M>() * 4;
+    return lines > p2 ? (lines > p3a ? lines : p3a) : (p2 > p3a ? p2 : p3a);
+}
+
+// clears the counters and writes the arguments the persistent launch reads
+// (kernel arguments would stay live in registers across every item type)
+__global__ void k_fused_setup(FusedArgs fa, FusedArgs* dst)
+{
+    int* p = reinterpret_cast<int*>(fa.sync);
+    for (int i = threadIdx.x; i < (int)(sizeof(FusedSync) / sizeof(int)); i += blockDim.x) p[i] = 0;
+    if (threadIdx.x == 0) *dst = fa;
+}
+
+// Values the persistent loop uses are re-read from LDS for every item and
+// made wavefront-uniform (readfirstlane), so that nothing derived from the
+// arguments is loop-invariant: otherwise the compiler hoists every item kind's
+// address arithmetic and twiddle loads out of the loop and keeps them all live
+// at once.  Pointers come back as global-address-space pointers (they are
+// device allocations), so the item code keeps its scalar-base global loads.
+template <class T>
+__device__ __forceinline__ T* uni_global(T* p)
+{
+    typedef __attribute__((address_space(1))) T gT;
+    return (T*)(gT*)(unsigned long long)uni((long long)(unsigned long long)p);
+}
+
+__device__ __forceinline__ EngineArgs args_from_lds(const EngineArgs& s)
+{
+    EngineArgs a;
+    a.g0 = uni_global(s.g0);
+    a.g1 = uni_global(s.g1);
+    a.f0 = uni_global(s.f0);
+    a.f1 = uni_global(s.f1);
+    a.v0 = uni_global(s.v0);
+    a.v1 = uni_global(s.v1);
+    a.out0 = uni_global(s.out0);
+    a.out1 = uni_global(s.out1);
+    a.ux_out = uni_global(s.ux_out);
+    a.uy_out = uni_global(s.uy_out);
+    a.img_stride = uni(s.img_stride);
+    a.w_stride = uni(s.w_stride);
+    a.N = uni(s.N);
+    a.e = uni(s.e);
+    a.W = uni(s.W);
+    a.GT = uni_global(s.GT);
+    a.tw = uni_global(s.tw);
+    a.zcol = uni_global(s.zcol);
+    a.tiles = uni_global(s.tiles);
+    a.X1 = uni_global(s.X1);
+    a.X2 = uni_global(s.X2);
+    a.np1 = uni(s.np1);
+    a.np2 = uni(s.np2);
+    a.S2 = uni(s.S2);
+    a.halo2 = uni(s.halo2);
+    a.p2_rows = uni(s.p2_rows);
+    a.ry_log2 = uni(s.ry_log2);
+    return a;
+}
+
+// The loop's control values (claimed item, wait results) are uniform
+// (readfirstlane) and the loop has a single latch: with per-lane control flow
+// (an item index read from LDS, `continue` paths, a claim and a signal done by
+// thread 0 only) the compiler may split the loop by lane, leaving all lanes
+// but thread 0 in an inner loop that re-runs the last item and never reaches
+// the claim.
+template <int M, int OP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FUSED_WPE)))
+void k_fused(const FusedArgs* fap)
+{
+    static_assert(P3A_WAVES == 4, "the fused workgroup is four wavefronts");
+    using PL = FftPlan<M>;
+    using S = float2;
+    constexpr bool ADJ = OP == (int)Op::Adjoint;
+    __shared__ __align__(16) unsigned char smem[fused_lds_bytes<M>()];
+    __shared__ FusedArgs s_fa;
+    __shared__ int s_item, s_flag;
+    float2(*lines)[line_slots(M)] = reinterpret_cast<float2(*)[line_slots(M)]>(smem);
+    float2* band_lines = reinterpret_cast<float2*>(smem);
+    float* wx = reinterpret_cast<float*>(smem + P3A_WAVES * line_slots(M) * 8);
+
+    {
+        const int* src = reinterpret_cast<const int*>(fap);
+        int* dst = reinterpret_cast<int*>(&s_fa);
+        for (int k = threadIdx.x; k < (int)(sizeof(FusedArgs) / sizeof(int)); k += blockDim.x)
+            dst[k] = src[k];
+    }
+    __syncthreads();
+    FusedSync* sy = uni_global(s_fa.sync);
+    const int x = xcd_id();
+    int* q = sy->x[x][0];
+    int* c1 = sy->x[x][1];   // c1 + kSyncLine * s: slot s
+    int* c2 = sy->x[x][1 + kFusedSlotsMax];
+    int* c3 = sy->x[x][1 + 2 * kFusedSlotsMax];
+    const int K = uni(s_fa.K), L = uni(s_fa.L);
+    const int B = uni(s_fa.B);
+    const int nx = B > x ? (B - x + 7) / 8 : 0;  // images of this XCD
+    const int n1a = uni(s_fa.n1a), n2a = uni(s_fa.n2), bx1b = uni(s_fa.bx1b);
+    const int n1 = n1a + uni(s_fa.n1b), n2 = (ADJ ? 2 : 1) * n2a, n3 = uni(s_fa.n3);
+    const int ipr = n1 + n2 + n3;
+    const int total = (nx + 2 * L) * ipr;
+    const unsigned long long t_end = wall_clock64() + (unsigned long long)uni((long long)s_fa.budget);
+    for (int i = 0; nx > 0;) {
+        if (wave_index() == 0) {
+            int v = 0;
+            if (threadIdx.x == 0)
+                v = __hip_atomic_fetch_add(q, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            v = uni(v);
+            if (uni(sync_load(&sy->abort[0])) != 0 || wall_clock64() > t_end) v = -1;
+            s_item = v;
+        }
+        __syncthreads();
+        i = uni(s_item);
+        __syncthreads();  // every wavefront has read s_item before it is rewritten
+        if (i < 0 || i >= total) break;
+        const int r = i / ipr, k = i - r * ipr;
+        const int stage = k < n3 ? 3 : k < n3 + n2 ? 2 : 1;
+        const int j = r - (stage == 3 ? 2 * L : stage == 2 ? L : 0);
+        if (j >= 0 && j < nx) {  // else pipeline fill / drain: nothing to do
+            const int idx = stage == 3 ? k : stage == 2 ? k - n3 : k - n3 - n2;
+            const int g = j / K, s = j - g * K;
+            const int slot = K * x + s;
+            const EngineArgs a = args_from_lds(s_fa.a);
+            const int b = 8 * j + x;
+            const Tile t = ADJ ? Tile{b, 0, 0, 0, a.N, 0, a.N, 0}
+                               : Tile{b, 0, 0, a.e, a.N - a.e, a.e, a.N - a.e, 0};
+            if (stage == 1) {
+                if (fused_wait(sy, c2 + kSyncLine * s, n2 * g, t_end, &s_flag)) {
+                    if (idx < n1a)
+                        p1_cols_item<M, PL, 1, S>(a, t, slot, idx, lines);
+                    else
+                        p1_adj_src_item<M, PL, 1, S>(a, 1, t, slot, bx1b + idx - n1a, lines);
+                    fused_signal(c1 + kSyncLine * s);
+                }
+            } else if (stage == 2) {
+                if (fused_wait(sy, c1 + kSyncLine * s, n1 * (g + 1), t_end, &s_flag) &&
+                    fused_wait(sy, c3 + kSyncLine * s, n3 * g, t_end, &s_flag)) {
+                    if (idx < n2a)
+                        p2_conv_band_fused<M, PL, S>(a, t, slot, idx, band_lines);
+                    else
+                        p2_corr_band_fused<M, PL, S>(a, t, slot, idx - n2a, 1, 2, band_lines);
+                    fused_signal(c2 + kSyncLine * s);
+                }
+            } else {
+                if (fused_wait(sy, c2 + kSyncLine * s, n2 * (g + 1), t_end, &s_flag)) {
+                    if constexpr (ADJ)
+                        p3_adj_item<M, PL, 1, S, true>(a, 1, 2, t, slot, idx, lines, wx);
+                    else
+                        p3_fwd_item<M, PL, 1, S, true>(a, t, slot, idx, lines);
+                    fused_signal(c3 + kSyncLine * s);
+                }
+            }
         }
     }
 }
@@ -1237,6 +1711,101 @@ int engine_run(Op op, int M, int prec, const EngineArgs& a, int ntiles, int chun
         }
     }
     return (int)hipErrorInvalidValue;
+}
+
+size_t fused_sync_bytes() { return sizeof(FusedSync) + sizeof(FusedArgs); }
+
+bool fused_supported(Op op, int M, int N, int prec)
+{
+    return prec == 0 && (op == Op::Forward || op == Op::Adjoint) && (M == 256 || M == 512) &&
+           N <= M;
+}
+
+namespace {
+
+// persistent grid: as many workgroups as are resident at once, a multiple of 8
+template <int M, int OP>
+int fused_grid()
+{
+    static const int g = [] {
+        int dev = 0, cus = 0, per = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per, reinterpret_cast<const void*>(&k_fused<M, OP>), 256, 0) != hipSuccess)
+            return 0;
+        const int n = (cus * per) & ~7;
+        return n > 0 ? n : 8;
+    }();
+    return g;
+}
+
+template <int M, int OP>
+int launch_fused(const FusedArgs* fap, hipStream_t s)
+{
+    const int g = fused_grid<M, OP>();
+    if (g <= 0) return (int)hipErrorInvalidValue;
+    k_fused<M, OP><<<g, 256, 0, s>>>(fap);
+    return (int)hipGetLastError();
+}
+
+}  // namespace
+
+int engine_run_fused(Op op, int M, const EngineArgs& a, int B, void* sync, int slots, int lag,
+                     hipStream_t s, LaunchTimer* tm)
+{
+    if (!fused_supported(op, M, a.N, 0) || B < 1 || !sync || lag < 1 || slots <= lag ||
+        slots > kFusedSlotsMax)
+        return (int)hipErrorInvalidValue;
+    FusedArgs fa{};
+    fa.a = a;
+    fa.a.ry_log2 = 3;  // kFusedRY
+    fa.sync = static_cast<FusedSync*>(sync);
+    fa.K = slots;
+    fa.L = lag;
+    fa.B = B;
+    const bool adj = op == Op::Adjoint;
+    const int N = a.N, e = a.e, W = a.W;
+    fa.n1a = M / 8;
+    if (adj) {
+        const int lo = e >= 3 ? e / 4 : 0;              // first block with 4 bx + 3 >= e
+        const int hi = std::min(M / 4, (N - e + 3) / 4);  // blocks with 4 bx < N - e
+        fa.bx1b = lo;
+        fa.n1b = std::max(0, hi - lo);
+    }
+    fa.n2 = M / (2 * kFusedRY) + 1;
+    fa.n3 = adj ? (N + 2 * P3A_WAVES - 1) / (2 * P3A_WAVES) : (W + 3) / 4;
+    // wall-clock budget of a workgroup (100 MHz ticks): 2 s plus 0.1 ms per
+    // image, far above the few microseconds an image takes
+    fa.budget = 200000000ull + 10000ull * (unsigned long long)B;
+    FusedArgs* fap = reinterpret_cast<FusedArgs*>(static_cast<char*>(sync) + sizeof(FusedSync));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t st = hipSuccess;
+    if (tm) st = tm->record(s, &e0);
+    k_fused_setup<<<1, 256, 0, s>>>(fa, fap);
+    if (st == hipSuccess) st = hipGetLastError();
+    if (st == hipSuccess) {
+        int r;
+        if (M == 256)
+            r = adj ? launch_fused<256, (int)Op::Adjoint>(fap, s) : launch_fused<256, (int)Op::Forward>(fap, s);
+        else
+            r = adj ? launch_fused<512, (int)Op::Adjoint>(fap, s) : launch_fused<512, (int)Op::Forward>(fap, s);
+        st = (hipError_t)r;
+    }
+    if (tm && st == hipSuccess) {
+        st = tm->record(s, &e1);
+        tm->recs.push_back({kFused, B, e0, e1});
+    }
+    return (int)st;
+}
+
+int fused_sync_status(const void* sync, int* timeouts)
+{
+    int h[2] = {0, 0};
+    const hipError_t st = hipMemcpy(h, static_cast<const FusedSync*>(sync)->status, sizeof(h),
+                                    hipMemcpyDeviceToHost);
+    if (timeouts) *timeouts = h[0];
+    return (int)st;
 }
 
 int engine_import_spectra(int M, const float2* Gx_ref, const float2* Gy_ref, float scale,

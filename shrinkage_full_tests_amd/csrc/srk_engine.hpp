@@ -91,6 +91,7 @@ enum KernelKind : int {
     kP2Corr,       // k_p2_corr
     kP3Fwd,        // k_p3_fwd
     kP3Adj,        // k_p3_adj
+    kFused,        // k_fused (the three passes of one operator in one persistent launch)
     kKernelKinds
 };
 
@@ -132,6 +133,20 @@ struct Fork {
 // alternate between two lanes.  Returns a hipError_t.
 int engine_run(Op op, int M, int prec, const EngineArgs& args, int ntiles, int chunk,
                hipStream_t stream, LaunchTimer* timer = nullptr, const Fork* fork = nullptr);
+
+// Fused pipeline (one persistent launch per operator): Op::Forward and
+// Op::Adjoint in FP32 for one-tile images (N <= M, M = 256 or 512).  The
+// exchange buffers args.X1 / args.X2 are rings of 8 * slots slots (np1 / np2
+// planes each; slots per XCD in [lag + 1, kFusedSlotsMax], lag >= 1 rounds
+// between stages); sync holds fused_sync_bytes() bytes of device memory.
+// Returns a hipError_t.
+constexpr int kFusedSlotsMax = 4;
+bool fused_supported(Op op, int M, int N, int prec);
+size_t fused_sync_bytes();
+int engine_run_fused(Op op, int M, const EngineArgs& args, int B, void* sync, int slots, int lag,
+                     hipStream_t stream, LaunchTimer* timer = nullptr);
+// Timed-out waits of the last fused launch that used sync (waits for it).
+int fused_sync_status(const void* sync, int* timeouts);
 
 // Transpose + scale reference-layout spectra (Ghat[k_x*(M/2+1)+k_y]) into GT.
 int engine_import_spectra(int M, const float2* Gx_ref, const float2* Gy_ref, float scale,

@@ -97,9 +97,10 @@ def test_product_kernel_generation_matches_scipy(kw, sigma):
 
 def test_kernel_kind_names_and_timing_null_workspace():
     L = _lib.lib()
-    names = [L.srkKernelKindName(k).decode() for k in range(6)]
-    assert names == ["k_p1_cols", "k_p1_adj_src", "k_p2_conv", "k_p2_corr", "k_p3_fwd", "k_p3_adj"]
-    assert L.srkKernelKindName(6) == b"" and L.srkKernelKindName(-1) == b""
+    names = [L.srkKernelKindName(k).decode() for k in range(7)]
+    assert names == ["k_p1_cols", "k_p1_adj_src", "k_p2_conv", "k_p2_corr", "k_p3_fwd", "k_p3_adj",
+                     "k_fused"]
+    assert L.srkKernelKindName(7) == b"" and L.srkKernelKindName(-1) == b""
     assert L.srkWorkspaceSetTiming(None, 1) == -3
     assert L.srkWorkspaceKernelTimes(None, None, None, None) == -3
 
