@@ -187,6 +187,8 @@ def main():
                     help="skip the ragged 64-instance fwd+adj+tangent-adjoint point")
     ap.add_argument("--f-ne-g", action="store_true",
                     help="main point with distinct images for f (I) and g (E)")
+    ap.add_argument("--no-fused", action="store_true",
+                    help="skip the fused one-launch pipeline timing (one-tile images, M <= 512)")
     ap.add_argument("--no-stored-u", action="store_true",
                     help="skip the stored-activation (u kept by the forward) fwd+adj point")
     ap.add_argument("--no-cd", action="store_true",
@@ -288,7 +290,10 @@ def main():
         torch.cuda.synchronize()
         ms_ = start.elapsed_time(stop) / args.steps
         # per-kernel launch durations: the same steps run eagerly with every
-        # launch bracketed by HIP events on the launch stream
+        # launch bracketed by HIP events on the launch stream, on one lane (with
+        # two lanes the launches of alternate chunks overlap and each one's
+        # duration counts the other's work)
+        ws.set_tuning(concurrency=0)
         ws.set_timing(True)
         ws.kernel_times()
         torch.cuda.synchronize()
@@ -300,6 +305,7 @@ def main():
         ms_eager = start.elapsed_time(stop) / args.steps
         kt = ws.kernel_times()
         ws.set_timing(False)
+        ws.set_tuning(concurrency=1 if args.concurrency < 0 else args.concurrency)
         kernels = {}
         for name, (tot, cnt, tiles) in kt.items():
             if cnt:
@@ -406,6 +412,28 @@ def main():
                   "speedup_vs_recompute": round(ms / msu, 3),
                   "stored_activation_bytes": 8 * B * W * W}
         del gu, ux, uy
+
+    fused_out = None
+    if not args.no_fused and M == N and M in (256, 512):
+        # the opt-in fused one-launch pipeline (SRK_TUNE_FUSED) on the same step, eager
+        ws.set_precision("fp32")
+        ws.set_tuning(fused=1)
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        start.record(stream)
+        for _ in range(args.steps):
+            step()
+        stop.record(stream)
+        torch.cuda.synchronize()
+        msf = start.elapsed_time(stop) / args.steps
+        available, timeouts = ws.fused_status()
+        ws.set_tuning(fused=0)
+        fused_out = {"workload": "the same fwd+adj step with SRK_TUNE_FUSED = 1 (one persistent "
+                                 "launch per operator, exchange ring of 4 slots per XCD, "
+                                 "stage lag 2), FP32, eager",
+                     "ms_per_step": round(msf, 3), "speedup_vs_three_pass": round(ms / msf, 3),
+                     "timed_out_waits": timeouts}
 
     cd_out = None
     if not args.no_cd:
@@ -600,15 +628,16 @@ def main():
                           "frac": kernels[dom]["pass_frac"],
                           "note": "the pass's own compulsory bytes (its input planes and "
                                   "images, its output planes) per launch / its average "
-                                  "launch duration; launches of the two lanes overlap"}
+                                  "launch duration, launches timed on one lane"}
                          if dom and "pass_GBps" in kernels.get(dom, {}) else None)},
-        "ms_per_step_eager_timed": round(ms_eager, 3) if ms_eager else None,
+        "ms_per_step_eager_timed_one_lane": round(ms_eager, 3) if ms_eager else None,
         "kernels": kernels,
         "mixed_fp16": reduced.get("mixed_fp16"),
         "bf16_exchange": reduced.get("bf16_exchange"),
         "secondary": secondary,
         "ragged": ragged_out,
         "stored_u": stored,
+        "fused_pipeline": fused_out,
         "cd_metric": cd_out,
         "reference_config_forward": refcfg,
         "cpu_baseline": cpu,

@@ -21,7 +21,9 @@ namespace {
 template <class PL>
 struct IsMfmaPlan { static constexpr bool value = false; };
 template <int M>
-struct IsMfmaPlan<FftPlanMfma<M>> { static constexpr bool value = true; };
+struct IsMfmaPlan<FftPlanMfma<M, true>> { static constexpr bool value = true; };
+template <int M>
+struct IsMfmaPlan<FftPlanMfma<M, false>> { static constexpr bool value = true; };
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 // wavefront index in the block, as a scalar (per-wave pointers stay in SGPRs)
@@ -408,7 +410,11 @@ struct P2Items {
             count = 1;
         }
     }
-    // item k of this block -> tile slot, band, field
+    // item k of this block -> tile slot, band, field.  An XCD takes its bands
+    // in groups of G consecutive ones (G xcd + r + 8 G m, r < G) when G divides
+    // its share, so that the G band segments of each 128-byte line of an X2
+    // column are written through the same L2 (G = p2_group below).
+    template <int G>
     __device__ void get(int k, int& slot, int& band, int& fi) const
     {
         const int i = first + k;
@@ -419,11 +425,19 @@ struct P2Items {
             band = NBF;
         } else {
             slot = j / BPX;
-            band = xcd + 8 * (j % BPX);
+            const int q = j % BPX;
+            band = BPX % G == 0 ? G * xcd + q % G + 8 * G * (q / G) : xcd + 8 * q;
         }
     }
     __device__ int rows(int band) const { return band < NBF ? RY : 1; }
 };
+
+// Bands whose segments (RY elements of S) fill one 128-byte line of an X2 column.
+template <int RY, class S>
+constexpr int p2_group()
+{
+    return RY * (int)sizeof(S) >= 128 ? 1 : 128 / (RY * (int)sizeof(S));
+}
 
 // LDS pitch of the P2 band lines: line_slots(M) + 16 - 16/RY, so that the
 // transposing accesses below (lane -> (row lane % RY, column lane / RY)) are
@@ -550,10 +564,10 @@ void k_p2_conv(EngineArgs a, int nf, int tiles)
     };
     BandRegs<M, RY, NW, S> pf;
     int slot, band, fi;
-    items.get(0, slot, band, fi);
+    items.template get<p2_group<RY, S>()>(0, slot, band, fi);
     pf.fetch(band_src(slot, band, fi), items.rows(band));
     for (int k = 0; k < items.count; ++k) {
-        items.get(k, slot, band, fi);
+        items.template get<p2_group<RY, S>()>(k, slot, band, fi);
         const int k0 = band * RY;
         const int nrows = items.rows(band);
         if (k) __syncthreads();  // the previous item's stores have read A and Bs
@@ -561,7 +575,7 @@ void k_p2_conv(EngineArgs a, int nf, int tiles)
         __syncthreads();
         if (k + 1 < items.count) {
             int s2, b2, f2;
-            items.get(k + 1, s2, b2, f2);
+            items.template get<p2_group<RY, S>()>(k + 1, s2, b2, f2);
             pf.fetch(band_src(s2, b2, f2), items.rows(b2));
         }
         const long long pe = (long long)a.S2 * HP;
@@ -662,7 +676,7 @@ void k_p2_corr(EngineArgs a, int planeIn, int planeOut, int tiles)
     // of the second block per CU
     int slot, band, fi;
     for (int k = 0; k < items.count; ++k) {
-        items.get(k, slot, band, fi);
+        items.template get<p2_group<RY, S>()>(k, slot, band, fi);
         const int k0 = band * RY;
         const int nrows = items.rows(band);
         const Tile t = a.tiles[slot];

@@ -9,12 +9,13 @@
 //   rows = butterflies (16 per tile), K = (re, im) of the 16 inputs,
 //   N = (re or im) of the 16 outputs (two products per tile),
 //
-// with FP16 operands, FP32 accumulation and FP32 twiddles.  Every FP32
-// operand x and every DFT coefficient c is split into FP16 parts
+// with FP16 operands, FP32 accumulation and FP32 twiddles.  SPLIT: every
+// FP32 operand x and every DFT coefficient c is split into FP16 parts
 // x = x_hi + x_lo, c = c_hi + c_lo, and each tile accumulates
 // x_hi c_hi + x_lo c_hi + x_hi c_lo (three MFMAs): about 22 significant bits
-// per operand instead of 11, so the transform error stays near FP32 level
-// while the arithmetic runs on the matrix cores.  Lane layout of
+// per operand instead of 11, so the transform error stays near FP32 level.
+// Otherwise one product x_hi c_hi per tile (FP16 rounding of operands and
+// coefficients, errors near 2^-11 of the line's magnitude).  Lane layout of
 // the 16x16x32 instruction (checked on the device): A[i][k] in lane i + 16*(k/8),
 // element k%8; B[k][n] in lane n + 16*(k/8), element k%8; D[i][n] in lane
 // n + 16*(i/4), element i%4.  K index k = 8q + e: q = 0, 1 carry the real parts
@@ -40,15 +41,17 @@ __device__ __forceinline__ float4_t mfma16(const half8_t& a, const half8_t& b, c
 }
 
 // B operand tiles of the forward 16-point DFT, split into FP16 hi and lo
-// parts: b[p][0] -> Re X_n, b[p][1] -> Im X_n (p = 0 hi, 1 lo).  The values
-// come from a 64-lane table built on the host (dft16_tile_table()).
+// parts: b[p][0] -> Re X_n, b[p][1] -> Im X_n (p = 0 hi, 1 lo; NP = 1 keeps
+// the hi parts only).  The values come from a 64-lane table built on the host
+// (srkWorkspaceCreate).
+template <int NP>
 struct Dft16Tiles {
-    half8_t b[2][2];
+    half8_t b[NP][2];
 
     __device__ void init(const half8_t* table, int lane)
     {
 #pragma unroll
-        for (int p = 0; p < 2; ++p)
+        for (int p = 0; p < NP; ++p)
 #pragma unroll
             for (int nb = 0; nb < 2; ++nb) b[p][nb] = table[(lane * 2 + p) * 2 + nb];
     }
@@ -57,37 +60,43 @@ struct Dft16Tiles {
 // One radix-16 MFMA Stockham pass with span Ns.  If !LAST the outputs are
 // multiplied by tw_next (the next pass's twiddles, per lane); if LAST they are
 // multiplied by `post` and conjugated when CONJ_OUT.
-template <int M, int Ns, bool LAST, bool CONJ_OUT>
-__device__ __forceinline__ void mfma16_pass(float2* __restrict__ line, const Dft16Tiles& T,
+template <int M, int Ns, bool LAST, bool CONJ_OUT, bool SPLIT>
+__device__ __forceinline__ void mfma16_pass(float2* __restrict__ line,
+                                            const Dft16Tiles<SPLIT ? 2 : 1>& T,
                                             const float2* tw_next, float post, int lane)
 {
     constexpr int NB = M / 16;    // butterflies
     constexpr int RB = NB / 16;   // row blocks
     const int i = lane & 15, q = lane >> 4;
     const bool im_part = q >= 2;
-    half8_t Ahi[RB], Alo[RB];
+    half8_t Ahi[RB], Alo[SPLIT ? RB : 1];
+    const float* lf = reinterpret_cast<const float*>(line) + (im_part ? 1 : 0);
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int j = 16 * rb + i;
             const int r = 8 * (q & 1) + e;
-            const float2 x = line[lslot<M>(j + r * NB)];
-            const float v = im_part ? x.y : x.x;
+            const float v = lf[2 * lslot<M>(j + r * NB)];
             const _Float16 hi = (_Float16)v;
             Ahi[rb][e] = hi;
-            Alo[rb][e] = (_Float16)(v - (float)hi);
+            if constexpr (SPLIT) Alo[rb][e] = (_Float16)(v - (float)hi);
         }
     float4_t Dre[RB], Dim[RB];
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) {
         const float4_t z = {0.f, 0.f, 0.f, 0.f};
-        float4_t re = mfma16(Alo[rb], T.b[0][0], z);
-        float4_t im = mfma16(Alo[rb], T.b[0][1], z);
-        re = mfma16(Ahi[rb], T.b[1][0], re);
-        im = mfma16(Ahi[rb], T.b[1][1], im);
-        Dre[rb] = mfma16(Ahi[rb], T.b[0][0], re);
-        Dim[rb] = mfma16(Ahi[rb], T.b[0][1], im);
+        if constexpr (SPLIT) {
+            float4_t re = mfma16(Alo[rb], T.b[0][0], z);
+            float4_t im = mfma16(Alo[rb], T.b[0][1], z);
+            re = mfma16(Ahi[rb], T.b[1][0], re);
+            im = mfma16(Ahi[rb], T.b[1][1], im);
+            Dre[rb] = mfma16(Ahi[rb], T.b[0][0], re);
+            Dim[rb] = mfma16(Ahi[rb], T.b[0][1], im);
+        } else {
+            Dre[rb] = mfma16(Ahi[rb], T.b[0][0], z);
+            Dim[rb] = mfma16(Ahi[rb], T.b[0][1], z);
+        }
     }
     // lane holds butterflies j = 16 rb + 4 q + e (e < 4), output s = i
 #pragma unroll
@@ -106,7 +115,7 @@ __device__ __forceinline__ void mfma16_pass(float2* __restrict__ line, const Dft
         }
 }
 
-template <int M>
+template <int M, bool SPLIT = true>
 struct FftPlanMfma {
     static constexpr int V = M / kWave;      // elements per lane
     static constexpr int R0 = M / 256;       // vector-ALU first pass radix (1 = none)
@@ -115,7 +124,7 @@ struct FftPlanMfma {
     static constexpr int LOG_R0 = R0 == 1 ? 0 : R0 == 2 ? 1 : R0 == 4 ? 2 : 3;
     static_assert(M == 256 * R0 && (R0 == 1 || R0 == 2 || R0 == 4), "plan covers 256, 512, 1024");
 
-    Dft16Tiles T;
+    Dft16Tiles<SPLIT ? 2 : 1> T;
     float2 tw0[R0 > 1 ? V : 1];   // for outputs of the radix-R0 pass (next span R0)
     float2 twA[NOUT];             // for outputs of the first radix-16 pass (next span 16 R0)
 
@@ -202,11 +211,11 @@ struct FftPlanMfma {
                     line[lslot<M>(R0 * j + r)] = cmul(y[r], tw0[g * R0 + r]);
             }
         }
-        mfma16_pass<M, R0, false, false>(line, T, twA, 1.0f, lane);
+        mfma16_pass<M, R0, false, false, SPLIT>(line, T, twA, 1.0f, lane);
         if (DIR > 0)
-            mfma16_pass<M, 16 * R0, true, true>(line, T, nullptr, post, lane);
+            mfma16_pass<M, 16 * R0, true, true, SPLIT>(line, T, nullptr, post, lane);
         else
-            mfma16_pass<M, 16 * R0, true, false>(line, T, nullptr, post, lane);
+            mfma16_pass<M, 16 * R0, true, false, SPLIT>(line, T, nullptr, post, lane);
     }
 };
 
