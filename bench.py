@@ -70,6 +70,18 @@ def pass_bytes_per_tile(N, M, e):
     }
 
 
+FP32_VECTOR_PEAK_TFS = 157.3   # MI355X FP32 vector (packed); 78.6 without packed FP32 ops
+
+
+def nominal_flops_per_tile(M):
+    """Nominal FLOPs of one tile through one forward and one adjoint (u
+    recomputed): 5 n log2 n per complex n-point FFT, real-input transforms
+    counted half, plus the spectrum products; fwd = 15 M^2 log2 M + 6 M^2,
+    adj = 30 M^2 log2 M + 12 M^2."""
+    lg = M.bit_length() - 1
+    return 45 * M * M * lg + 18 * M * M
+
+
 def ref_train_ws_bytes(N, B):
     """Device bytes of the reference's batched forward workspace
     (SrkTrainWorkspace::allocate, training/srk_training_forward.cuh:130-141): five
@@ -189,6 +201,8 @@ def main():
                     help="main point with distinct images for f (I) and g (E)")
     ap.add_argument("--no-fused", action="store_true",
                     help="skip the fused one-launch pipeline timing (one-tile images, M <= 512)")
+    ap.add_argument("--no-fwd-configs", action="store_true",
+                    help="skip the forward-only sweep over the batched training shapes")
     ap.add_argument("--no-stored-u", action="store_true",
                     help="skip the stored-activation (u kept by the forward) fwd+adj point")
     ap.add_argument("--no-cd", action="store_true",
@@ -413,6 +427,53 @@ def main():
                   "stored_activation_bytes": 8 * B * W * W}
         del gu, ux, uy
 
+    fwd_cfg = None
+    if not args.no_fwd_configs and (N, B, kw) == (512, 5000, 121):
+        # forward-only Mpix/s over the batched training shapes (B = 5000,
+        # N in {256, 512}, KW in {31, 61, 121}, E = I; dx changes no work),
+        # srkTrainingForward replayed from a hipGraph
+        fwd_cfg = {"workload": "srkTrainingForward, B=5000, E=I, FP32, one hipGraph replay "
+                               "per call", "points": []}
+        ws.set_precision("fp32")
+        for Nf in (256, 512):
+            Ef = E if Nf == N else patterns.grf_torch(B, Nf, seed=1100)
+            for kwf in (31, 61, 121):
+                ef = (kwf - 1) // 2
+                Wf = Nf - 2 * ef
+                gxf, gyf = srk.kernel_spectra_srsrk(kwf, ef / 3.0, Nf)
+                wsf = srk.Workspace(Nf, Wf, ef, Nf, B_max=B)
+                Rf = torch.empty((B, Wf, Wf), device="cuda")
+
+                def ff():
+                    srk.srkTrainingForward(Ef, Ef, gxf, gyf, Nf, Wf, ef, dx, B, wsf, Rf)
+
+                ff()
+                torch.cuda.synchronize()
+                gf = torch.cuda.CUDAGraph()
+                sf = torch.cuda.Stream()
+                sf.wait_stream(torch.cuda.current_stream())
+                with torch.cuda.stream(sf):
+                    with torch.cuda.graph(gf, stream=sf):
+                        ff()
+                torch.cuda.current_stream().wait_stream(sf)
+                gf.replay()
+                torch.cuda.synchronize()
+                start.record(stream)
+                for _ in range(5):
+                    gf.replay()
+                stop.record(stream)
+                torch.cuda.synchronize()
+                msf_ = start.elapsed_time(stop) / 5
+                bf = (4 * Nf * Nf + 4 * Wf * Wf) * B   # E = I: read the image once, write R
+                fwd_cfg["points"].append({
+                    "N": Nf, "KW": kwf, "W": Wf, "ms_per_call": round(msf_, 3),
+                    "Mpix_per_s": round(B * Nf * Nf / (msf_ / 1e3) / 1e6, 1),
+                    "valid_Mpix_per_s": round(B * Wf * Wf / (msf_ / 1e3) / 1e6, 1),
+                    "hbm_frac": round(bf / (msf_ / 1e3) / 1e9 / HBM_PEAK_GBS, 4)})
+                del gf, wsf, Rf
+            del Ef
+        torch.cuda.empty_cache()
+
     fused_out = None
     if not args.no_fused and M == N and M in (256, 512):
         # the opt-in fused one-launch pipeline (SRK_TUNE_FUSED) on the same step, eager
@@ -604,6 +665,17 @@ def main():
                    "tiles_per_chunk_fwd": info["chunk_fwd"], "tiles_per_chunk_adj": info["chunk_adj"],
                    "hipgraph": graph,
                    "precision": args.precision if args.precision in reduced else "fp32"},
+        "valid_Mpix_per_s": round(B * W * W / (ms / 1e3) / 1e6, 1),
+        "flops": ({"convention": "nominal: 5 n log2 n per complex n-point FFT, real-input "
+                                 "transforms half, spectrum products; fwd+adj per tile "
+                                 "45 M^2 log2 M + 18 M^2",
+                   "TFLOP_per_step": round(nominal_flops_per_tile(M) * B / 1e12, 3),
+                   "achieved_TFLOP_per_s": round(nominal_flops_per_tile(M) * B / (ms / 1e3)
+                                                 / 1e12, 2),
+                   "fp32_vector_peak_TFLOP_per_s": FP32_VECTOR_PEAK_TFS,
+                   "frac": round(nominal_flops_per_tile(M) * B / (ms / 1e3) / 1e12
+                                 / FP32_VECTOR_PEAK_TFS, 4)}
+                  if M == N else None),
         "ms_fwd": round(t_f, 3),
         "ms_adj": round(t_a, 3),
         "memory": memory,
@@ -638,6 +710,7 @@ def main():
         "ragged": ragged_out,
         "stored_u": stored,
         "fused_pipeline": fused_out,
+        "forward_configs": fwd_cfg,
         "cd_metric": cd_out,
         "reference_config_forward": refcfg,
         "cpu_baseline": cpu,

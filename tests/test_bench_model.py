@@ -51,3 +51,11 @@ def test_reference_training_workspace_size():
     # reports 6.63 GB in use there, cuFFT work areas included)
     assert bench.ref_train_ws_bytes(512, 5000) == 36741120000
     assert abs(bench.ref_train_ws_bytes(256, 3000) / 1e9 - 5.52) < 0.01
+
+
+def test_nominal_flop_convention():
+    # 5 n log2 n per complex FFT; a fwd+adj tile is 9 M-line FFT sweeps' worth
+    # (fwd 3, adj 6) of M^2 log2 M plus the products: 45 M^2 log2 M + 18 M^2
+    M = 512
+    assert bench.nominal_flops_per_tile(M) == 45 * M * M * 9 + 18 * M * M
+    assert bench.nominal_flops_per_tile(256) == 45 * 256 * 256 * 8 + 18 * 256 * 256
