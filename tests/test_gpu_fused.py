@@ -126,3 +126,71 @@ def test_fused_ring_limits():
     ws.set_tuning(fused_slots=2, fused_lag=1)
     with pytest.raises(srk.SrkError):
         ws.set_tuning(fused_slots=1)  # not above the lag
+
+
+def test_fused_single_image_with_u_outputs():
+    """srkForwardOaS through the fused launch (one-tile image) writes h, u_x and
+    u_y equal to the three-pass path's."""
+    import torch
+    N, kw, M = 300, 61, 512
+    e = (kw - 1) // 2
+    W = N - 2 * e
+    print(f"\n[fused u] N={N} M={M}", flush=True)
+    gx, gy = srk.kernel_spectra_srsrk(kw, e / 3.0, M)
+    E = patterns.grf_torch(1, N, seed=41)[0].contiguous()
+    I = patterns.grf_torch(1, N, seed=42)[0].contiguous()
+    ws = srk.Workspace(N, W, e, M)
+    out = {}
+    for fused in (0, 1):
+        ws.set_tuning(fused=fused)
+        R, ux, uy = (torch.full((W, W), float("nan"), device="cuda") for _ in range(3))
+        srk.srkForwardOaS(E, I, gx, gy, N, W, e, 1.0, ws, R, ux, uy)
+        torch.cuda.synchronize()
+        out[fused] = (R, ux, uy)
+    assert ws.fused_status() == (True, 0)
+    for a, b in zip(out[1], out[0]):
+        assert not torch.isnan(a).any()
+        assert (a - b).abs().max().item() <= 1e-5 * b.abs().max().item()
+
+
+def test_fused_graph_replays_match_eager():
+    """The fused launch (its argument-setup kernel and the persistent kernel)
+    captured in a hipGraph: several replays, with eager work between them,
+    each equal to the eager result."""
+    import torch
+    N, kw, M, B = 256, 31, 256, 24
+    e = (kw - 1) // 2
+    W = N - 2 * e
+    print(f"\n[fused graph] N={N} M={M} B={B}", flush=True)
+    gx, gy = srk.kernel_spectra_srsrk(kw, e / 3.0, M)
+    E = patterns.grf_torch(B, N, seed=51)
+    v = torch.randn((B, W, W), device="cuda", generator=torch.Generator("cuda").manual_seed(52))
+    ws = srk.Workspace(N, W, e, M, B_max=B)
+    ws.set_tuning(fused=1)
+    R = torch.empty((B, W, W), device="cuda")
+    lf = torch.empty((B, N, N), device="cuda")
+    lg = torch.empty((B, N, N), device="cuda")
+
+    def step():
+        srk.forward_batched(E, E, gx, gy, e, 1.0, ws, R)
+        srk.adjoint_batched(v, E, E, gx, gy, e, 1.0, ws, lf, lg)
+
+    step()
+    torch.cuda.synchronize()
+    ref = [R.clone(), lf.clone(), lg.clone()]
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            step()
+    torch.cuda.current_stream().wait_stream(s)
+    for it in range(3):
+        R.zero_(); lf.zero_(); lg.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for got, want in zip((R, lf, lg), ref):
+            assert torch.equal(got, want), f"replay {it} differs from eager"
+        _ = (E * 2).sum().item()   # eager work between replays
+    assert ws.fused_status() == (True, 0)
