@@ -203,6 +203,8 @@ def main():
                     help="skip the fused one-launch pipeline timing (one-tile images, M <= 512)")
     ap.add_argument("--no-fwd-configs", action="store_true",
                     help="skip the forward-only sweep over the batched training shapes")
+    ap.add_argument("--no-derivatives", action="store_true",
+                    help="skip the single-image derivative operators (N 1024-4096, KW 61)")
     ap.add_argument("--no-stored-u", action="store_true",
                     help="skip the stored-activation (u kept by the forward) fwd+adj point")
     ap.add_argument("--no-cd", action="store_true",
@@ -576,9 +578,51 @@ def main():
                                "hbm_frac": round(bL / (msM / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
                                "speedup_vs_fp32": round(msL / msM, 3)}
 
+    deriv = None
+    if not args.no_derivatives and (N, B, kw) == (512, 5000, 121):
+        # the differentiated single-image path of the derivative checks
+        # (tan_tanadj/main_tan_tanadj.cu): forward, adjoint (u recomputed),
+        # tangent and tangent-adjoint, B = 1, KW = 61, f != g, M = 1024 tiles
+        if not args.no_large:
+            del fL, gL, vL, hL, lfL, lgL, wsL
+        torch.cuda.empty_cache()
+        deriv = {"workload": "B=1, KW=61 (erosion 30), f != g GRF, M=1024, FP32, eager calls",
+                 "points": []}
+        kwd = 61
+        ed = (kwd - 1) // 2
+        gxd, gyd = srk.kernel_spectra_srsrk(kwd, ed / 3.0, 1024)
+        for Nd in (1024, 2048, 4096):
+            Wd = Nd - 2 * ed
+            f1, g1, df1, dg1 = (patterns.grf_torch(1, Nd, seed=7000 + 10 * Nd + i) for i in range(4))
+            vd = torch.randn((1, Wd, Wd), generator=gen, device="cuda")
+            wd = torch.randn((1, Wd, Wd), generator=gen, device="cuda")
+            wsd = srk.Workspace(Nd, Wd, ed, 1024, B_max=1)
+            ops = {
+                "forward": lambda: srk.forward_batched(g1, f1, gxd, gyd, ed, dx, wsd),
+                "adjoint": lambda: srk.adjoint_batched(vd, g1, f1, gxd, gyd, ed, dx, wsd),
+                "tangent": lambda: srk.tangent_batched(f1, g1, df1, dg1, gxd, gyd, ed, dx, wsd),
+                "tangent_adjoint": lambda: srk.tangent_adjoint_batched(
+                    f1, g1, df1, dg1, vd, wd, gxd, gyd, ed, dx, wsd),
+            }
+            pt = {"N": Nd}
+            for name, op in ops.items():
+                op()
+                torch.cuda.synchronize()
+                start.record(stream)
+                for _ in range(5):
+                    op()
+                stop.record(stream)
+                torch.cuda.synchronize()
+                msd = start.elapsed_time(stop) / 5
+                pt[name] = {"ms": round(msd, 3), "Mpix_per_s": round(Nd * Nd / (msd / 1e3) / 1e6, 1)}
+            deriv["points"].append(pt)
+            wsd.close()
+            del f1, g1, df1, dg1, vd, wd
+        torch.cuda.empty_cache()
+
     ragged_out = None
     if not args.no_ragged:
-        if not args.no_large:
+        if not args.no_large and deriv is None:
             del fL, gL, vL, hL, lfL, lgL, wsL
         torch.cuda.empty_cache()
         from shrinkage_full_tests_amd import ragged
@@ -711,6 +755,7 @@ def main():
         "stored_u": stored,
         "fused_pipeline": fused_out,
         "forward_configs": fwd_cfg,
+        "derivatives": deriv,
         "cd_metric": cd_out,
         "reference_config_forward": refcfg,
         "cpu_baseline": cpu,
