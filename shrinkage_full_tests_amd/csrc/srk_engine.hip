@@ -129,12 +129,15 @@ __device__ __forceinline__ Bf16c pack<Bf16c>(float2 v)
 }
 template <class S>
 __device__ __forceinline__ float2 ldx(const S* base, unsigned idx) { return unpack(ldu(base, idx)); }
-// NT: a streaming store (the three-pass path); the fused pipeline keeps its
-// exchange rings cached, since it wants them to stay in the L2
+// NT: a streaming store (the three-pass path) of an FP32 element; the
+// fused pipeline keeps its exchange rings cached, since it wants them to stay
+// in the L2.  bfloat16 elements are stored cached: their band segments are
+// 32-byte pieces of a line, which streaming stores wrote out unmerged (P2
+// conv 7.5 -> 10.1 ms in the bfloat16-exchange step at N = 512)
 template <bool NT, class S>
 __device__ __forceinline__ void stx(S* base, unsigned idx, float2 v)
 {
-    if constexpr (NT)
+    if constexpr (NT && sizeof(S) == 8)
         stn(base, idx, pack<S>(v));
     else
         stu(base, idx, pack<S>(v));
@@ -540,7 +543,7 @@ __device__ __forceinline__ void store_band(S* __restrict__ plane, const float2* 
         const int x = lo + xi;
         float2 v = make_float2(0.f, 0.f);
         if (x >= 0 && x < M) v = src[lslot<M>(x)];
-        if constexpr (NT)
+        if constexpr (NT && sizeof(S) == 8)  // FP32 only, as stx
             stn(d, 0u, pack<S>(v));
         else
             *d = pack<S>(v);
