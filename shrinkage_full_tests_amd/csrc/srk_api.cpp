@@ -322,17 +322,20 @@ size_t lane_capacity(const srk_workspace_s* w, int np1, int np2, int S2, int lan
 }
 
 // Two lanes (each with half of the exchange buffers) when pipelining is on,
-// the call's ntiles tiles take more than one chunk of the whole buffers (or
+// the call's ntiles tiles take at least four chunks of the whole buffers (or
 // of the chunk the caller set) and half of the buffers holds at least one
-// tile; otherwise one.  A call that fits one chunk runs it on the caller's
-// stream: split over two lanes, a small batch pays for two sets of launches
-// and the fork and join (N = 512: 125 against 51 us for a two-image adjoint).
+// tile; otherwise one.  Split over two lanes, a call pays for twice the
+// launches, smaller chunks and the fork and join: a two-image adjoint at
+// N = 512 took 125 against 51 us, and the N = 16384 fwd+adj step (two and
+// three chunks per call) 9.50 against 9.40 ms; from four chunks on the
+// overlap wins (N = 512, B = 5000: 30.04 against 30.29 ms).
 int lane_count(const srk_workspace_s* w, int np1, int np2, int S2, int ntiles)
 {
     size_t one = lane_capacity(w, np1, np2, S2, 1);
     if (w->chunk_override > 0) one = std::min<size_t>(one, (size_t)w->chunk_override);
-    return w->concurrent && (size_t)ntiles > one && lane_capacity(w, np1, np2, S2, 2) >= 1 ? 2
-                                                                                          : 1;
+    return w->concurrent && (size_t)ntiles > 3 * one && lane_capacity(w, np1, np2, S2, 2) >= 1
+               ? 2
+               : 1;
 }
 
 int chunk_for(const srk_workspace_s* w, int np1, int np2, int S2, int ntiles)
