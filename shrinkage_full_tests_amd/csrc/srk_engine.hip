@@ -1018,11 +1018,82 @@ __device__ __forceinline__ void p3_fwd_item(const EngineArgs& a, const Tile& t, 
     }
 }
 
+// P3 forward with one field, as p3_fwd_item, but with the image columns of
+// the gradient stencils shared through LDS.  After its transform a
+// wavefront's line is free: it takes the wavefront's own image column at
+// every tile row, neighbours read it there instead of loading columns gx - 1
+// and gx + 1 again, and the y-differences come from the same line; only the
+// block's edge wavefronts load an outer neighbour column from memory.  The
+// valid rows of a tile lie at least one row inside it, so y - 1 and y + 1 are
+// tile rows.  Same arithmetic as p3_fwd_item: bit-identical results.
+template <int M, class PL, class S>
+__device__ __forceinline__ void p3_fwd_shared_item(const EngineArgs& a, const Tile& t, int slot,
+                                                   int bx, float2 (*lds)[line_slots(M)])
+{
+    constexpr int V = M / kWave;
+    constexpr int HP = spec_pitch(M);
+    const int lane = lane_id(), w = wave_id();
+    const int cnt = t.ox1 - t.ox0;
+    const int xi = bx * 4 + w;
+    const bool active = xi < cnt;  // the active wavefronts of a block are a prefix
+    const int N = a.N, e = a.e, W = a.W;
+    const int gx = t.ox0 + xi;
+    float* fline = reinterpret_cast<float*>(lds[w]);
+    const float* fc = a.f0 + (long long)t.b * a.img_stride + (long long)gx * N;
+    const long long ob = (long long)t.b * a.w_stride + (long long)(gx - e) * W - e;
+    float2 u0[M / kWave];
+    if (active) {
+        PL FP;
+        FP.load(a.tw, lane);
+        const long long pe = (long long)a.S2 * HP;
+        const S* P = plane_at<const S>(a.X2, (long long)slot * a.np2, pe) + (long long)xi * HP;
+        const S* Q = plane_at<const S>(a.X2, (long long)slot * a.np2 + 1, pe) + (long long)xi * HP;
+        ifft_pair<M, PL, S>(FP, lds[w], P, Q, u0, lane);
+        // rows outside the image load a clamped row; no valid row reads them
+#pragma unroll
+        for (int j = 0; j < V; ++j) {
+            const int y = lane + kWave * j;
+            fline[y] = ldu(fc, (unsigned)min(max(t.p0y + y, 0), N - 1));
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    const float* fleft = w > 0 ? reinterpret_cast<const float*>(lds[w - 1]) : nullptr;
+    const float* fright =
+        w < 3 && xi + 1 < cnt ? reinterpret_cast<const float*>(lds[w + 1]) : nullptr;
+    float* out = a.out0 + ob;  // indexed by the global row
+#pragma unroll
+    for (int j = 0; j < V; ++j) {
+        const int y = lane + kWave * j;
+        const int gy = t.p0y + y;
+        if (gy >= t.oy0 && gy < t.oy1) {
+            const float l = fleft ? fleft[y] : ldu(fc - N, (unsigned)gy);
+            const float r = fright ? fright[y] : ldu(fc + N, (unsigned)gy);
+            const float2 g = make_float2(0.5f * (r - l), 0.5f * (fline[y + 1] - fline[y - 1]));
+            stu(out, (unsigned)gy, u0[j].x * g.x + u0[j].y * g.y);
+        }
+    }
+    if (a.ux_out) {
+#pragma unroll
+        for (int j = 0; j < V; ++j) {
+            const int gy = t.p0y + lane + kWave * j;
+            if (gy >= t.oy0 && gy < t.oy1) {
+                stu(a.ux_out + ob, (unsigned)gy, u0[j].x);
+                stu(a.uy_out + ob, (unsigned)gy, u0[j].y);
+            }
+        }
+    }
+}
+
 template <int M, class PL, int NF, class S>
-__global__ __launch_bounds__(256) void k_p3_fwd(EngineArgs a)
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(M <= 512 && NF == 1 ? 8 : 1))) void k_p3_fwd(EngineArgs a)
 {
     __shared__ float2 lds[4][line_slots(M)];
-    p3_fwd_item<M, PL, NF, S>(a, a.tiles[blockIdx.y], blockIdx.y, blockIdx.x, lds);
+    if constexpr (NF == 1)
+        p3_fwd_shared_item<M, PL, S>(a, a.tiles[blockIdx.y], blockIdx.y, blockIdx.x, lds);
+    else
+        p3_fwd_item<M, PL, NF, S>(a, a.tiles[blockIdx.y], blockIdx.y, blockIdx.x, lds);
 }
 
 // ---------------------------------------------------------------------------
