@@ -1048,13 +1048,20 @@ __device__ __forceinline__ void p3_fwd_shared_item(const EngineArgs& a, const Ti
         const long long pe = (long long)a.S2 * HP;
         const S* P = plane_at<const S>(a.X2, (long long)slot * a.np2, pe) + (long long)xi * HP;
         const S* Q = plane_at<const S>(a.X2, (long long)slot * a.np2 + 1, pe) + (long long)xi * HP;
-        ifft_pair<M, PL, S>(FP, lds[w], P, Q, u0, lane);
-        // rows outside the image load a clamped row; no valid row reads them
+        // the image column's loads go out once the spectrum values are packed
+        // (their raw registers are free then) and land during the transform;
+        // rows outside the image load a clamped row, which no valid row reads
+        PairRaw<M, S> raw;
+        raw.load(P, Q, lane);
+        float2 z[M / kWave];
+        raw.pack(z, lane);
+        float fcol[M / kWave];
 #pragma unroll
-        for (int j = 0; j < V; ++j) {
-            const int y = lane + kWave * j;
-            fline[y] = ldu(fc, (unsigned)min(max(t.p0y + y, 0), N - 1));
-        }
+        for (int j = 0; j < V; ++j)
+            fcol[j] = ldu(fc, (unsigned)min(max(t.p0y + lane + kWave * j, 0), N - 1));
+        fft_regs_to_regs<M, +1>(FP, lds[w], z, u0, lane);
+#pragma unroll
+        for (int j = 0; j < V; ++j) fline[lane + kWave * j] = fcol[j];
     }
     __syncthreads();
     if (!active) return;
