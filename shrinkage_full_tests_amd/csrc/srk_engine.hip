@@ -1194,14 +1194,17 @@ __device__ __forceinline__ void p3_adj_item(const EngineArgs& a, int with_u, int
                     for (int q = 0; q < V; ++q) u[q] = make_float2(0.f, 0.f);
                 }
             };
+            // both own columns' cotangents load before the first transform
+            float cv1[V];
             cot(gx, cv);
+            cot(gx + 1, cv1);
             u_pair(j0);
 #pragma unroll
             for (int j = 0; j < V; ++j) {
                 wx[(2 * w + 1) * M + lane + kWave * j] = u[j].x * cv[j];
                 wy0[j] = u[j].y * cv[j];
+                cv[j] = cv1[j];
             }
-            cot(gx + 1, cv);
             u_pair(j0 + 1);
 #pragma unroll
             for (int j = 0; j < V; ++j) {
