@@ -149,6 +149,46 @@ __device__ __forceinline__ S* plane_at(float2* X, long long index, long long ele
     return reinterpret_cast<S*>(X + index * elems);
 }
 
+// X2 plane layout ([x][k_y], column pitch HP = spec_pitch(M)): row k_y < M/2
+// of column xi lies at xi HP + k_y, and the Nyquist rows (k_y = M/2) of the
+// sixteen columns 16 g .. 16 g + 15 share the padding after the rows of
+// column 16 g, at 16 g HP + M/2 + (xi - 16 g).  An FP32 column then fills 16
+// whole 128-byte lines plus a sixteenth of a shared one, instead of 16 lines
+// and a 17th that holds one element.
+//
+// Tiles of M >= 1024 keep each column's Nyquist element after its rows (at
+// xi HP + M/2): their columns span 32 lines, and the grouped layout made the
+// M = 1024 correlation pass slower.
+template <int M>
+constexpr bool x2_grouped() { return M <= 512; }
+
+// element index of row k <= M/2 of column xi, relative to the plane
+template <int M>
+__device__ __forceinline__ long long x2_index(int xi, int k)
+{
+    return k == M / 2 && x2_grouped<M>()
+               ? (long long)(xi & ~15) * spec_pitch(M) + M / 2 + (xi & 15)
+               : (long long)xi * spec_pitch(M) + k;
+}
+
+// A column as the P3 passes address it: its rows k < M/2 from c, its Nyquist
+// element at n; the zero column (spec_pitch(M) zeros) is {zcol, zcol}.
+template <class S>
+struct XCol {
+    const S* c;
+    const S* n;
+};
+template <int M, class S>
+__device__ __forceinline__ XCol<S> x2_col(const S* plane, int xi)
+{
+    return {plane + (long long)xi * spec_pitch(M), plane + x2_index<M>(xi, M / 2)};
+}
+template <class S>
+__device__ __forceinline__ XCol<S> zero_col(const float2* zcol)
+{
+    return {reinterpret_cast<const S*>(zcol), reinterpret_cast<const S*>(zcol)};
+}
+
 // Two Hermitian half spectra P, Q (X2 columns, or the zero column) as loaded
 // for the lane's transform inputs k = lane + 64 i (raw, so that a load can be
 // issued one transform ahead), and their packing Z = P + iQ.  The DC and
@@ -158,14 +198,23 @@ struct PairRaw {
     static constexpr int V = M / kWave;
     S p[V], q[V];
 
-    __device__ __forceinline__ void load(const S* P, const S* Q, int lane)
+    __device__ __forceinline__ void load(const XCol<S>& P, const XCol<S>& Q, int lane)
     {
+        constexpr int IN = M / 2 / kWave;  // k = M/2 at i = IN, lane 0
 #pragma unroll
         for (int i = 0; i < V; ++i) {
             const int k = lane + kWave * i;
-            const unsigned kk = k <= M / 2 ? k : M - k;
-            p[i] = ldc<COH>(P, kk);
-            q[i] = ldc<COH>(Q, kk);
+            // rows k < M/2 directly, k > M/2 by Hermitian symmetry from M - k;
+            // lane 0 of i = IN (k = M/2) reads row M/2 - 1 with lane 1 (same
+            // line) and then its Nyquist element
+            constexpr bool G = x2_grouped<M>();
+            const unsigned kk = i < IN ? k : (G && i == IN && lane == 0 ? M / 2 - 1 : M - k);
+            p[i] = ldc<COH>(P.c, kk);
+            q[i] = ldc<COH>(Q.c, kk);
+            if (G && i == IN && lane == 0) {
+                p[i] = ldc<COH>(P.n, 0u);
+                q[i] = ldc<COH>(Q.n, 0u);
+            }
         }
     }
     __device__ __forceinline__ void pack(float2 (&v)[V], int lane) const
@@ -187,8 +236,8 @@ struct PairRaw {
 // Inverse transform of P + iQ; afterwards out[i] = p(y) + i q(y) at tile row
 // y = lane + 64 i.
 template <int M, class PL, class S, bool COH = false>
-__device__ __forceinline__ void ifft_pair(const PL& FP, float2* line, const S* P, const S* Q,
-                                          float2 (&out)[M / kWave], int lane)
+__device__ __forceinline__ void ifft_pair(const PL& FP, float2* line, const XCol<S>& P,
+                                          const XCol<S>& Q, float2 (&out)[M / kWave], int lane)
 {
     PairRaw<M, S, COH> r;
     r.load(P, Q, lane);
@@ -533,7 +582,7 @@ struct BandRegs {
     }
 };
 
-// LDS lines -> X2 plane ([x][k_y], pitch HP): X2 column xi holds tile-local
+// LDS lines -> X2 plane (layout at x2_col): X2 column xi holds tile-local
 // column lo + xi; columns xi in [xs, xe) are stored, those outside [0, M) as
 // zero.
 template <int M, int RY, int NW, class S, bool NT = true>
@@ -544,6 +593,17 @@ __device__ __forceinline__ void store_band(S* __restrict__ plane, const float2* 
     const int r = threadIdx.x % RY, xi0 = xs + threadIdx.x / RY;
     if (r >= nrows) return;
     const float2* src = L + r * LP;
+    if (x2_grouped<M>() && k0 == M / 2) {
+        // the Nyquist row: sixteen columns' elements fill one line, written by
+        // several wavefronts, so the stores stay cached (merged in the L2)
+        for (int xi = xi0; xi < xe; xi += XS) {
+            const int x = lo + xi;
+            float2 v = make_float2(0.f, 0.f);
+            if (x >= 0 && x < M) v = src[lslot<M>(x)];
+            plane[x2_index<M>(xi, M / 2)] = pack<S>(v);
+        }
+        return;
+    }
     S* d = plane + (long long)xi0 * HP + k0 + r;
     for (int xi = xi0; xi < xe; xi += XS, d += (long long)XS * HP) {
         const int x = lo + xi;
@@ -970,8 +1030,8 @@ __device__ __forceinline__ void p3_fwd_item(const EngineArgs& a, const Tile& t, 
     float2* line = lds[w];
     // this column of X2 plane p
     auto col = [&](int p) {
-        return plane_at<const S>(a.X2, (long long)slot * a.np2 + p, (long long)a.S2 * HP) +
-               (long long)xi * HP;
+        return x2_col<M>(plane_at<const S>(a.X2, (long long)slot * a.np2 + p, (long long)a.S2 * HP),
+                         xi);
     };
     // output rows [oy0, oy1) and the column lie inside the valid region, so
     // every gradient stencil is inside the image
@@ -1046,8 +1106,8 @@ __device__ __forceinline__ void p3_fwd_shared_item(const EngineArgs& a, const Ti
         PL FP;
         FP.load(a.tw, lane);
         const long long pe = (long long)a.S2 * HP;
-        const S* P = plane_at<const S>(a.X2, (long long)slot * a.np2, pe) + (long long)xi * HP;
-        const S* Q = plane_at<const S>(a.X2, (long long)slot * a.np2 + 1, pe) + (long long)xi * HP;
+        const XCol<S> P = x2_col<M>(plane_at<const S>(a.X2, (long long)slot * a.np2, pe), xi);
+        const XCol<S> Q = x2_col<M>(plane_at<const S>(a.X2, (long long)slot * a.np2 + 1, pe), xi);
         // the image column's loads go out once the spectrum values are packed
         // (their raw registers are free then) and land during the transform;
         // rows outside the image load a clamped row, which no valid row reads
@@ -1150,15 +1210,16 @@ __device__ __forceinline__ void p3_adj_item(const EngineArgs& a, int with_u, int
     auto colS = [&](auto fmt, int plane, int j) {
         using T = decltype(fmt);
         const int xi2 = j + 1;
-        return xi2 < ncols ? plane_at<const T>(a.X2, (long long)slot * a.np2 + plane, pstride) +
-                                 (long long)xi2 * HP
-                           : reinterpret_cast<const T*>(a.zcol);
+        return xi2 < ncols
+                   ? x2_col<M>(plane_at<const T>(a.X2, (long long)slot * a.np2 + plane, pstride),
+                               xi2)
+                   : zero_col<T>(a.zcol);
     };
     // u planes: stored only for global columns [e, N - e) (u_columns)
     auto col = [&](int plane, int j) {
         const int c = t.ox0 + j;
         return c >= a.e && c < a.N - a.e ? colS(float2{}, plane, j)
-                                         : static_cast<const float2*>(a.zcol);
+                                         : zero_col<float2>(a.zcol);
     };
     const long long ib = (long long)t.b * a.img_stride;
 
