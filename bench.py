@@ -538,6 +538,8 @@ def main():
         specL = {ML: srk.kernel_spectra_srsrk(kw, sigma, ML) for ML in tiles_L}
         wsL = {ML: srk.Workspace(NL, WL, e, ML, B_max=1) for ML in tiles_L}
 
+        eager_ms = {}  # (precision, M) -> eager ms per step
+
         def time_large(precision, ML):
             w_, (gxl, gyl) = wsL[ML], specL[ML]
             w_.set_precision(precision)
@@ -546,6 +548,8 @@ def main():
                 srk.forward_batched(gL, fL, gxl, gyl, e, dx, w_, hL)
                 srk.adjoint_batched(vL, gL, fL, gxl, gyl, e, dx, w_, lfL, lgL)
 
+            # timed as the main point: a hipGraph of the step, replayed (the
+            # eager time is kept beside it in eager_ms)
             stepL()
             torch.cuda.synchronize()
             start.record(stream)
@@ -553,7 +557,26 @@ def main():
                 stepL()
             stop.record(stream)
             torch.cuda.synchronize()
-            return start.elapsed_time(stop) / 3
+            eager_ms[(precision, ML)] = start.elapsed_time(stop) / 3
+            if args.no_graph:
+                return eager_ms[(precision, ML)]
+            gr = torch.cuda.CUDAGraph()
+            s = torch.cuda.Stream()
+            s.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(s):
+                with torch.cuda.graph(gr, stream=s):
+                    stepL()
+            torch.cuda.current_stream().wait_stream(s)
+            gr.replay()
+            torch.cuda.synchronize()
+            reps = max(3, args.steps)
+            start.record(stream)
+            for _ in range(reps):
+                gr.replay()
+            stop.record(stream)
+            torch.cuda.synchronize()
+            del gr
+            return start.elapsed_time(stop) / reps
 
         def best_large(precision):
             by_m = {ML: time_large(precision, ML) for ML in tiles_L
@@ -568,6 +591,8 @@ def main():
         secondary = {"workload": f"fwd+adj N={NL} B=1 KW={kw} f!=g",
                      "value": round(NL * NL / (msL / 1e3) / 1e6, 1), "unit": "Mpix/s",
                      "ms_per_step": round(msL, 3), "M": MLb, "ms_by_M": byL,
+                     "ms_per_step_eager": round(eager_ms[("fp32", MLb)], 3),
+                     "timing": "hipGraph replay" if not args.no_graph else "eager",
                      "hbm_frac": round(bL / (msL / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
         for prec in reduced:
             msM, MLm, byM = best_large(prec)
@@ -575,6 +600,7 @@ def main():
                 continue
             secondary[prec] = {"value": round(NL * NL / (msM / 1e3) / 1e6, 1),
                                "ms_per_step": round(msM, 3), "M": MLm, "ms_by_M": byM,
+                               "ms_per_step_eager": round(eager_ms[(prec, MLm)], 3),
                                "hbm_frac": round(bL / (msM / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
                                "speedup_vs_fp32": round(msL / msM, 3)}
 

@@ -141,8 +141,8 @@ int srkCheckSpectraSupport(const void* Gx_freq, const void* Gy_freq, int Ns, int
  * (training/srk_training_forward.cuh:117-158).  One workspace serves every
  * entry point for images of size N with the given erosion, batches up to
  * B_max, FFT/tile size M (the size of the spectra passed to the calls; M = N
- * for srkTrainingForward).  scratch_mb bounds the exchange buffers (0 = 4096; allocations never
- * exceed what B_max images need).
+ * for srkTrainingForward).  scratch_mb bounds the exchange buffers (0 = 4096, or up to 16384
+ * when the tiles of one image need more; allocations never exceed what B_max images need).
  */
 int srkWorkspaceCreate(int N, int W, int erosion, int M, int B_max, size_t scratch_mb,
                        srk_workspace_t* ws);

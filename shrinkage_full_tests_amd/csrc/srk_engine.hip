@@ -565,12 +565,17 @@ struct BandRegs {
         constexpr int XS = T / RY;
         const bool ok = (int)(threadIdx.x % RY) < nrows;
         const int x0 = threadIdx.x / RY;
-        const S* src = band + threadIdx.x;
+        // a full band is x-major with RY rows per x; the Nyquist band (one
+        // row) is contiguous along x
+        const bool nyq = nrows == 1;
+        const S* src = band + (nyq ? x0 : (int)threadIdx.x);
+        const int step = nyq ? XS : T;
 #pragma unroll
         for (int i = 0; i < IT; ++i) {
             const int x = x0 + i * XS;
             v[i] = pack<S>(make_float2(0.f, 0.f));
-            if (ok && x >= xlo && x < xhi) v[i] = COH ? ldc<true>(src, i * T) : ldn(src, i * T);
+            if (ok && x >= xlo && x < xhi)
+                v[i] = COH ? ldc<true>(src, i * step) : ldn(src, i * step);
         }
     }
     __device__ __forceinline__ void put(float2* L) const

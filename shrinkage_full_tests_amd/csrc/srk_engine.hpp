@@ -67,12 +67,12 @@ struct EngineArgs {
 constexpr __host__ __device__ int spec_pitch(int M) { return M / 2 + 16; }
 
 // X1 plane index of (k_y, x) in band layout with bands of 2^rs rows: bands of
-// M * 2^rs elements, x-major inside a band.  The last band (k_y = M/2 only)
-// ends at (M/2 + 2^rs) * M <= M * spec_pitch(M) for 2^rs <= 16.
+// M * 2^rs elements, x-major inside a band.  The last band holds the Nyquist
+// row k_y = M/2 alone, contiguous along x: M/2 * M + x.
 template <int M>
 __host__ __device__ constexpr int x1_index(int k, int x, int rs)
 {
-    return (((k >> rs) * M + x) << rs) + (k & ((1 << rs) - 1));
+    return k == M / 2 ? M / 2 * M + x : (((k >> rs) * M + x) << rs) + (k & ((1 << rs) - 1));
 }
 
 enum class Op : int {
