@@ -13,6 +13,7 @@
 #include "srk_fft_mfma.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace srk {
 
@@ -189,6 +190,18 @@ __device__ __forceinline__ XCol<S> zero_col(const float2* zcol)
     return {reinterpret_cast<const S*>(zcol), reinterpret_cast<const S*>(zcol)};
 }
 
+// Lane (64 - lane) mod 64's value of v (a cross-lane read through the LDS
+// crossbar; no LDS memory is used).
+__device__ __forceinline__ float2 mirror_lane(float2 v, int addr)
+{
+    return make_float2(__int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v.x))),
+                       __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v.y))));
+}
+__device__ __forceinline__ Bf16c mirror_lane(Bf16c v, int addr)
+{
+    return {(unsigned)__builtin_amdgcn_ds_bpermute(addr, (int)v.v)};
+}
+
 // Two Hermitian half spectra P, Q (X2 columns, or the zero column) as loaded
 // for the lane's transform inputs k = lane + 64 i (raw, so that a load can be
 // issued one transform ahead), and their packing Z = P + iQ.  The DC and
@@ -233,13 +246,68 @@ struct PairRaw {
     }
 };
 
+// As PairRaw, with half the loads and registers: only the rows
+// k < M/2 (lane + 64 i, i < M/128) and the Nyquist row are loaded; the
+// transform inputs k > M/2 take P[M - k] from the lane that loaded it:
+// M - k = (64 - lane) + 64 (V - 1 - i) for lane > 0, and 64 (V - i) (lane 0's
+// own row, or the Nyquist row for i = M/128) for lane 0.  The DC and Nyquist
+// bins are taken as real, as a real inverse transform does.
+template <int M, class S, bool COH = false>
+struct PairHalf {
+    static constexpr int V = M / kWave;
+    static constexpr int IN = M / 2 / kWave;  // k = M/2 at i = IN, lane 0
+    S p[IN], q[IN];
+    S pn, qn;  // Nyquist rows
+
+    __device__ __forceinline__ void load(const XCol<S>& P, const XCol<S>& Q, int lane)
+    {
+#pragma unroll
+        for (int i = 0; i < IN; ++i) {
+            p[i] = ldc<COH>(P.c, (unsigned)(lane + kWave * i));
+            q[i] = ldc<COH>(Q.c, (unsigned)(lane + kWave * i));
+        }
+        pn = ldc<COH>(P.n, 0u);
+        qn = ldc<COH>(Q.n, 0u);
+    }
+    __device__ __forceinline__ void pack(float2 (&v)[V], int lane) const
+    {
+        const int addr = ((kWave - lane) & (kWave - 1)) * 4;
+#pragma unroll
+        for (int i = 0; i < IN; ++i) {
+            float2 pp = unpack(p[i]), qq = unpack(q[i]);
+            if (i == 0 && lane == 0) {  // DC
+                pp.y = 0.f;
+                qq.y = 0.f;
+            }
+            v[i] = make_float2(pp.x - qq.y, pp.y + qq.x);
+        }
+#pragma unroll
+        for (int i = IN; i < V; ++i) {
+            float2 pp = unpack(mirror_lane(p[V - 1 - i], addr));
+            float2 qq = unpack(mirror_lane(q[V - 1 - i], addr));
+            if (lane == 0) {
+                pp = unpack(i == IN ? pn : p[V - i]);
+                qq = unpack(i == IN ? qn : q[V - i]);
+                if (i == IN) {  // Nyquist
+                    pp.y = 0.f;
+                    qq.y = 0.f;
+                }
+            }
+            v[i] = make_float2(pp.x + qq.y, qq.x - pp.y);
+        }
+    }
+};
+
 // Inverse transform of P + iQ; afterwards out[i] = p(y) + i q(y) at tile row
 // y = lane + 64 i.
-template <int M, class PL, class S, bool COH = false>
+// HALF: PairHalf loads (the P3 adjoint: N = 512 step 29.64 -> 29.25 ms); the
+// P3 forward keeps PairRaw (its 64-register bound at M = 512 spills with the
+// lane exchange).
+template <int M, class PL, class S, bool COH = false, bool HALF = false>
 __device__ __forceinline__ void ifft_pair(const PL& FP, float2* line, const XCol<S>& P,
                                           const XCol<S>& Q, float2 (&out)[M / kWave], int lane)
 {
-    PairRaw<M, S, COH> r;
+    std::conditional_t<HALF, PairHalf<M, S, COH>, PairRaw<M, S, COH>> r;
     r.load(P, Q, lane);
     float2 v[M / kWave];
     r.pack(v, lane);
@@ -1254,7 +1322,7 @@ __device__ __forceinline__ void p3_adj_item(const EngineArgs& a, int with_u, int
             auto u_pair = [&](int j) {
                 const int c = t.ox0 + j;
                 if (c >= e && c < N - e) {
-                    ifft_pair<M, PL, float2, COH>(FP, line, col(px, j), col(py, j), u, lane);
+                    ifft_pair<M, PL, float2, COH, true>(FP, line, col(px, j), col(py, j), u, lane);
                 } else {
 #pragma unroll
                     for (int q = 0; q < V; ++q) u[q] = make_float2(0.f, 0.f);
@@ -1294,8 +1362,8 @@ __device__ __forceinline__ void p3_adj_item(const EngineArgs& a, int with_u, int
                 cot(cl, cv);
                 cot(cr0, cr);
                 if ((cl >= e && cl < N - e) || (cr0 >= e && cr0 < N - e)) {
-                    ifft_pair<M, PL, float2, COH>(FP, line, col(px, jb - 1), col(px, jb + CBK), u,
-                                                  lane);
+                    ifft_pair<M, PL, float2, COH, true>(FP, line, col(px, jb - 1),
+                                                        col(px, jb + CBK), u, lane);
                 } else {
 #pragma unroll
                     for (int q = 0; q < V; ++q) u[q] = make_float2(0.f, 0.f);
@@ -1332,8 +1400,8 @@ __device__ __forceinline__ void p3_adj_item(const EngineArgs& a, int with_u, int
     if (j0 < cnt) {
         const bool second = j0 + 1 < cnt;
         float2 l[V];
-        ifft_pair<M, PL, SL, COH>(FP, line, colS(SL{}, planeL, j0), colS(SL{}, planeL, j0 + 1), l,
-                                  lane);
+        ifft_pair<M, PL, SL, COH, true>(FP, line, colS(SL{}, planeL, j0),
+                                        colS(SL{}, planeL, j0 + 1), l, lane);
         float* o1 = a.out1 + ib + (long long)gx * N;
 #pragma unroll
         for (int j = 0; j < V; ++j) {
