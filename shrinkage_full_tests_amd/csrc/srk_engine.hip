@@ -300,9 +300,7 @@ struct PairHalf {
 
 // Inverse transform of P + iQ; afterwards out[i] = p(y) + i q(y) at tile row
 // y = lane + 64 i.
-// HALF: PairHalf loads (the P3 adjoint: N = 512 step 29.64 -> 29.25 ms); the
-// P3 forward keeps PairRaw (its 64-register bound at M = 512 spills with the
-// lane exchange).
+// HALF: PairHalf loads (the P3 adjoint: N = 512 step 29.60 -> 28.89 ms).
 template <int M, class PL, class S, bool COH = false, bool HALF = false>
 __device__ __forceinline__ void ifft_pair(const PL& FP, float2* line, const XCol<S>& P,
                                           const XCol<S>& Q, float2 (&out)[M / kWave], int lane)
@@ -1184,7 +1182,10 @@ __device__ __forceinline__ void p3_fwd_shared_item(const EngineArgs& a, const Ti
         // the image column's loads go out once the spectrum values are packed
         // (their raw registers are free then) and land during the transform;
         // rows outside the image load a clamped row, which no valid row reads
-        PairRaw<M, S> raw;
+        // M <= 512: half loads with the lane exchange (PairHalf), at seven
+        // wavefronts per SIMD (67 registers; P3 forward 3.98 -> 3.83 ms at
+        // N = 512, B = 5000); M = 1024 keeps PairRaw, which measured faster there
+        std::conditional_t<(M <= 512), PairHalf<M, S>, PairRaw<M, S>> raw;
         raw.load(P, Q, lane);
         float2 z[M / kWave];
         raw.pack(z, lane);
@@ -1227,7 +1228,7 @@ __device__ __forceinline__ void p3_fwd_shared_item(const EngineArgs& a, const Ti
 
 template <int M, class PL, int NF, class S>
 __global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(M <= 512 && NF == 1 ? 8 : 1))) void k_p3_fwd(EngineArgs a)
+__attribute__((amdgpu_waves_per_eu(M <= 512 && NF == 1 ? 7 : 1))) void k_p3_fwd(EngineArgs a)
 {
     __shared__ float2 lds[4][line_slots(M)];
     if constexpr (NF == 1)
