@@ -544,10 +544,14 @@ __global__ __launch_bounds__(256) void k_p1_adj_src(EngineArgs a, int plane0)
 // XCD reads only its own eighth of the spectra, which stays resident in its
 // L2.  Each such block takes up to ITEMS consecutive items of its XCD's
 // list (the conv pass prefetches the next item's band into registers while it
-// transforms the current one).  The Nyquist rows (k_y = M/2, a band of one
+// transforms the current one, and in band-major order keeps the band's
+// spectrum rows in registers).  The Nyquist rows (k_y = M/2, a band of one
 // row) follow, one item per block.
 // ---------------------------------------------------------------------------
-template <int M, int RY, int ITEMS>
+// BM (band-major): a block's consecutive items are one band of consecutive
+// tiles (the conv pass keeps that band's spectrum rows in registers across
+// them); otherwise consecutive bands of one tile.
+template <int M, int RY, int ITEMS, bool BM = false>
 struct P2Items {
     static constexpr int NBF = M / (2 * RY);  // full bands per tile
     static constexpr int BPX = NBF / 8;       // full bands per tile and XCD
@@ -558,9 +562,9 @@ struct P2Items {
     }
     static constexpr int blocks(int tiles, int nf) { return full_blocks(tiles, nf) + tiles * nf; }
 
-    int xcd, first, count, nf;
+    int xcd, first, count, nf, tiles;
     bool nyquist;
-    __device__ P2Items(int tiles, int nf_) : nf(nf_)
+    __device__ P2Items(int tiles_, int nf_) : nf(nf_), tiles(tiles_)
     {
         const int L = blockIdx.x;
         const int fb = full_blocks(tiles, nf);
@@ -589,8 +593,8 @@ struct P2Items {
             slot = j;
             band = NBF;
         } else {
-            slot = j / BPX;
-            const int q = j % BPX;
+            slot = BM ? j % tiles : j / BPX;
+            const int q = BM ? j / tiles : j % BPX;
             band = BPX % G == 0 ? G * xcd + q % G + 8 * G * (q / G) : xcd + 8 * q;
         }
     }
@@ -705,8 +709,14 @@ constexpr bool p2_one_line()
 {
     return !IsMfmaPlan<PL>::value && M >= 1024;
 }
+// Items per conv block: with two lines per row (M <= 512) the FP32 plans take
+// four items of one band (spectrum rows loaded once; N = 512 step 28.73 ->
+// 28.58 ms, where two items per block, band or tile major, tie)
 template <int M, class PL>
-constexpr int p2c_items() { return p2_one_line<M, PL, kP2ConvPass>() ? 1 : 2; }
+constexpr int p2c_items()
+{
+    return p2_one_line<M, PL, kP2ConvPass>() ? 1 : IsMfmaPlan<PL>::value ? 2 : 4;
+}
 
 // X2 columns [xs, xe) of a tile's u planes that P3 reads: u enters the
 // operators only multiplied by a field that vanishes off the valid region
@@ -735,13 +745,18 @@ void k_p2_conv(EngineArgs a, int nf, int tiles)
     constexpr int LP = p2_pitch<M, RY>();
     constexpr bool ONE = p2_one_line<M, PL, kP2ConvPass>();
     static_assert(!ONE || NW == RY, "one band row per wavefront");
+    // with one band row per wavefront and two lines per row, the row's
+    // spectra stay in registers while the block's items share the band
+    constexpr bool KEEP = !ONE && NW == RY && !IsMfmaPlan<PL>::value;
     extern __shared__ float2 sm[];
     float2* A = sm;
     float2* Bs = ONE ? sm : sm + RY * LP;
     const int lane = lane_id(), w = wave_id();
     PL FP;
     FP.load(a.tw, lane);
-    const P2Items<M, RY, p2c_items<M, PL>()> items(tiles, nf);
+    const P2Items<M, RY, p2c_items<M, PL>(), KEEP> items(tiles, nf);
+    float2 gxv[V], gyv[V];
+    int gband = -1;  // band whose spectrum rows gxv, gyv hold (KEEP)
     auto band_src = [&](int slot, int band, int fi) {
         return plane_at<const S>(a.X1, (long long)slot * a.np1 + fi, (long long)M * HP) +
                (long long)band * RY * M;
@@ -806,11 +821,13 @@ void k_p2_conv(EngineArgs a, int nf, int tiles)
             float2* lb = Bs + r * LP;
             const float2* gxr = a.GT + (long long)(k0 + r) * M + lane;
             const float2* gyr = a.GT + (long long)(H + k0 + r) * M + lane;
-            float2 gxv[V], gyv[V];
+            if (!KEEP || band != gband) {
 #pragma unroll
-            for (int i = 0; i < V; ++i) {
-                gxv[i] = ldu(gxr, kWave * i);
-                gyv[i] = ldu(gyr, kWave * i);
+                for (int i = 0; i < V; ++i) {
+                    gxv[i] = ldu(gxr, kWave * i);
+                    gyv[i] = ldu(gyr, kWave * i);
+                }
+                gband = band;
             }
             float2 X[V], U[V];
             fft_lds_to_regs<M, -1>(FP, la, X, lane);
