@@ -1185,25 +1185,36 @@ __device__ __forceinline__ void p3_fwd_shared_item(const EngineArgs& a, const Ti
     const int xi = bx * 4 + w;
     const bool active = xi < cnt;  // the active wavefronts of a block are a prefix
     const int N = a.N, e = a.e, W = a.W;
-    const int gx = t.ox0 + xi;
     float* fline = reinterpret_cast<float*>(lds[w]);
-    const float* fc = a.f0 + (long long)t.b * a.img_stride + (long long)gx * N;
-    const long long ob = (long long)t.b * a.w_stride + (long long)(gx - e) * W - e;
+    // image column and output addresses: formed after the spectrum loads at
+    // M <= 512 (63 instead of 67 registers, eight wavefronts per SIMD), before
+    // them at M = 1024 (115 instead of 142 registers)
+    const int gx = t.ox0 + xi;
+    const float* fc = nullptr;
+    long long ob = 0;
+    auto addresses = [&] {
+        fc = a.f0 + (long long)t.b * a.img_stride + (long long)gx * N;
+        ob = (long long)t.b * a.w_stride + (long long)(gx - e) * W - e;
+    };
+    if constexpr (M > 512) addresses();
+    // M <= 512: half loads with the lane exchange (PairHalf; P3 forward 3.98
+    // -> 3.78 ms at N = 512, B = 5000); M = 1024 keeps PairRaw, which
+    // measured faster there
+    using Raw = std::conditional_t<(M <= 512), PairHalf<M, S>, PairRaw<M, S>>;
+    PL FP;
+    Raw raw;
+    if (active) {
+        const long long pe = (long long)a.S2 * HP;
+        FP.load(a.tw, lane);
+        raw.load(x2_col<M>(plane_at<const S>(a.X2, (long long)slot * a.np2, pe), xi),
+                 x2_col<M>(plane_at<const S>(a.X2, (long long)slot * a.np2 + 1, pe), xi), lane);
+    }
+    if constexpr (M <= 512) addresses();
     float2 u0[M / kWave];
     if (active) {
-        PL FP;
-        FP.load(a.tw, lane);
-        const long long pe = (long long)a.S2 * HP;
-        const XCol<S> P = x2_col<M>(plane_at<const S>(a.X2, (long long)slot * a.np2, pe), xi);
-        const XCol<S> Q = x2_col<M>(plane_at<const S>(a.X2, (long long)slot * a.np2 + 1, pe), xi);
         // the image column's loads go out once the spectrum values are packed
         // (their raw registers are free then) and land during the transform;
         // rows outside the image load a clamped row, which no valid row reads
-        // M <= 512: half loads with the lane exchange (PairHalf), at seven
-        // wavefronts per SIMD (67 registers; P3 forward 3.98 -> 3.83 ms at
-        // N = 512, B = 5000); M = 1024 keeps PairRaw, which measured faster there
-        std::conditional_t<(M <= 512), PairHalf<M, S>, PairRaw<M, S>> raw;
-        raw.load(P, Q, lane);
         float2 z[M / kWave];
         raw.pack(z, lane);
         float fcol[M / kWave];
@@ -1245,7 +1256,7 @@ __device__ __forceinline__ void p3_fwd_shared_item(const EngineArgs& a, const Ti
 
 template <int M, class PL, int NF, class S>
 __global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(M <= 512 && NF == 1 ? 7 : 1))) void k_p3_fwd(EngineArgs a)
+__attribute__((amdgpu_waves_per_eu(M <= 512 && NF == 1 ? 8 : 1))) void k_p3_fwd(EngineArgs a)
 {
     __shared__ float2 lds[4][line_slots(M)];
     if constexpr (NF == 1)
