@@ -703,11 +703,15 @@ __device__ __forceinline__ void store_band(S* __restrict__ plane, const float2* 
 // registers meanwhile, with one item per block (no prefetch registers).  At
 // M = 1024 (N = 16384 step) the one-line corr pass took 2.33-2.43 ms against
 // 2.80 for two lines; the conv pass measured the same either way (4.45-4.58 ms).
+// The corr pass keeps one line per row at M = 512 as well, at six wavefronts
+// per SIMD (80 registers, three 38 KB blocks per CU): 4.10 -> 3.87 ms in the
+// N = 512, B = 5000 step; the conv pass there keeps two lines and its
+// four-item blocks.
 enum P2Pass { kP2ConvPass = 0, kP2CorrPass = 1 };
 template <int M, class PL, int PASS>
 constexpr bool p2_one_line()
 {
-    return !IsMfmaPlan<PL>::value && M >= 1024;
+    return !IsMfmaPlan<PL>::value && (M >= 1024 || (PASS == kP2CorrPass && M == 512));
 }
 // Items per conv block: with two lines per row (M <= 512) the FP32 plans take
 // four items of one band (spectrum rows loaded once; N = 512 step 28.73 ->
@@ -732,7 +736,10 @@ __device__ __forceinline__ void u_columns(const EngineArgs& a, const Tile& t, in
 
 // two 8-wavefront blocks per CU at M = 1024 need at most 128 registers
 template <int M, class PL, int PASS>
-constexpr int p2_waves_per_eu() { return p2_one_line<M, PL, PASS>() && M == 1024 ? 4 : 1; }
+constexpr int p2_waves_per_eu()
+{
+    return !p2_one_line<M, PL, PASS>() ? 1 : M == 1024 ? 4 : M == 512 ? 6 : 1;
+}
 
 template <int M, int RY, int NW, class PL, class S>
 __global__ __launch_bounds__(NW * kWave)
