@@ -715,11 +715,19 @@ constexpr bool p2_one_line()
 }
 // Items per conv block: with two lines per row (M <= 512) the FP32 plans take
 // four items of one band (spectrum rows loaded once; N = 512 step 28.73 ->
-// 28.58 ms, where two items per block, band or tile major, tie)
-template <int M, class PL>
+// 28.58 ms, where two items per block, band or tile major, tie) when the
+// planes are FP32; bfloat16 planes keep two tile-major items, so that the
+// four bands whose 32-byte segments fill an X2 line are written together
+// (band-major, bf16-exchange P2 conv 7.3 -> 9.8 ms)
+template <int M, class PL, class S>
+constexpr bool p2c_keep()
+{
+    return !p2_one_line<M, PL, kP2ConvPass>() && !IsMfmaPlan<PL>::value && sizeof(S) == 8;
+}
+template <int M, class PL, class S>
 constexpr int p2c_items()
 {
-    return p2_one_line<M, PL, kP2ConvPass>() ? 1 : IsMfmaPlan<PL>::value ? 2 : 4;
+    return p2_one_line<M, PL, kP2ConvPass>() ? 1 : p2c_keep<M, PL, S>() ? 4 : 2;
 }
 
 // X2 columns [xs, xe) of a tile's u planes that P3 reads: u enters the
@@ -754,14 +762,14 @@ void k_p2_conv(EngineArgs a, int nf, int tiles)
     static_assert(!ONE || NW == RY, "one band row per wavefront");
     // with one band row per wavefront and two lines per row, the row's
     // spectra stay in registers while the block's items share the band
-    constexpr bool KEEP = !ONE && NW == RY && !IsMfmaPlan<PL>::value;
+    constexpr bool KEEP = p2c_keep<M, PL, S>() && NW == RY;
     extern __shared__ float2 sm[];
     float2* A = sm;
     float2* Bs = ONE ? sm : sm + RY * LP;
     const int lane = lane_id(), w = wave_id();
     PL FP;
     FP.load(a.tw, lane);
-    const P2Items<M, RY, p2c_items<M, PL>(), KEEP> items(tiles, nf);
+    const P2Items<M, RY, p2c_items<M, PL, S>(), KEEP> items(tiles, nf);
     float2 gxv[V], gyv[V];
     int gband = -1;  // band whose spectrum rows gxv, gyv hold (KEEP)
     auto band_src = [&](int slot, int band, int fi) {
@@ -1908,12 +1916,13 @@ int run_impl(Op op, const EngineArgs& a0, int ntiles, int chunk, hipStream_t s0,
             });
         };
         auto p2c = [&](int nf, bool x16, hipStream_t ls) {
-            const dim3 g2(P2Items<M, RY, p2c_items<M, PL>()>::blocks(c, nf));
+            const dim3 g2x(P2Items<M, RY, p2c_items<M, PL, SX>()>::blocks(c, nf));
+            const dim3 g2f(P2Items<M, RY, p2c_items<M, PL, float2>()>::blocks(c, nf));
             run(kP2Conv, ls, [&] {
                 if (x16)
-                    k_p2_conv<M, RY, NW, PL, SX><<<g2, b2, lds2c, ls>>>(a, nf, c);
+                    k_p2_conv<M, RY, NW, PL, SX><<<g2x, b2, lds2c, ls>>>(a, nf, c);
                 else
-                    k_p2_conv<M, RY, NW, PL, float2><<<g2, b2, lds2c, ls>>>(a, nf, c);
+                    k_p2_conv<M, RY, NW, PL, float2><<<g2f, b2, lds2c, ls>>>(a, nf, c);
             });
         };
         auto p2r = [&](int in, int out, hipStream_t ls) {
