@@ -263,6 +263,7 @@ struct srk_workspace_s {
     srk::Fork fork{nullptr, nullptr, nullptr, nullptr, nullptr};
     srk::Tile* d_ftiles = nullptr;
     srk::Tile* d_atiles = nullptr;
+    int src_b0 = 0, src_b1 = 0;  // P1A column blocks with valid-region columns (EngineArgs)
     int nft = 0, nat = 0, S2f = 0, S2a = 0;
     float2* X1 = nullptr;
     float2* X2 = nullptr;
@@ -548,6 +549,8 @@ int run_adj(srk_workspace_s* w, srk::Op op, srk::EngineArgs a, int B, hipStream_
     a.tiles = w->d_atiles;
     a.S2 = w->S2a;
     a.halo2 = 1;
+    a.src_b0 = w->src_b0;
+    a.src_b1 = w->src_b1;
     if (use_fused(w, op)) {
         a.X1 = w->X1f;
         a.X2 = w->X2f;
@@ -670,6 +673,16 @@ int srkWorkspaceCreate(int N, int W, int erosion, int M, int B_max, size_t scrat
     auto* w = new srk_workspace_s();
     w->N = N; w->W = W; w->e = erosion; w->M = M; w->B_max = B_max;
     w->nft = (int)ft.size(); w->nat = (int)at.size(); w->S2f = S2f; w->S2a = S2a;
+    // P1A column blocks of 4 tile columns that meet the valid region [e, N - e)
+    // in some adjoint tile
+    w->src_b0 = M / 4;
+    w->src_b1 = 0;
+    for (const srk::Tile& t : at) {
+        const int lo = std::max(erosion - t.p0x, 0), hi = std::min(N - erosion - t.p0x, M);
+        if (hi <= lo) continue;
+        w->src_b0 = std::min(w->src_b0, lo / 4);
+        w->src_b1 = std::max(w->src_b1, (hi + 3) / 4);
+    }
     // replicate the per-image tables over the batch
     std::vector<srk::Tile> fb((size_t)B_max * ft.size()), ab((size_t)B_max * at.size());
     for (int b = 0; b < B_max; ++b) {

@@ -60,6 +60,10 @@ struct EngineArgs {
     int halo2;             // extra stored columns on each side in X2 (0 fwd, 1 adj)
     int p2_rows;           // k_y bands per P2 block (tuning: 16, 8, 4; 0 = default)
     int ry_log2;           // log2 of the band height in effect (set by the engine)
+    // P1A column blocks (4 tile columns each) [src_b0, src_b1) that hold
+    // valid-region columns in some tile of the call; the P1A grid covers only
+    // these (src_b1 <= src_b0: all M / 4)
+    int src_b0, src_b1;
 };
 
 // Padded pitch (complex elements) of one column spectrum in X2, and the
