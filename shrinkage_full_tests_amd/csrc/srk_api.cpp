@@ -323,19 +323,19 @@ size_t lane_capacity(const srk_workspace_s* w, int np1, int np2, int S2, int lan
 }
 
 // Two lanes (each with half of the exchange buffers) when pipelining is on,
-// the call's ntiles tiles take at least eight chunks of the whole buffers (or
-// of the chunk the caller set) and half of the buffers holds at least one
+// the call's ntiles tiles take at least sixteen chunks of the whole buffers
+// (or of the chunk the caller set) and half of the buffers holds at least one
 // tile; otherwise one.  Split over two lanes, a call pays for twice the
 // launches, smaller chunks and the fork and join: a two-image adjoint at
 // N = 512 took 125 against 51 us, the N = 16384 fwd+adj step (two and three
 // chunks per call) 9.50 against 9.40 ms, and the N = 512, B = 5000 step
-// (adjoint in 4.4 chunks) 27.73 against 27.66 ms; at B = 10000 (adjoint in
-// 8.8 chunks) the overlap wins, 55.37 against 55.42 ms.
+// (forward in 3.6 chunks, adjoint in 8.8) 27.73 against 27.66 ms; at
+// B = 10000 (7.1 and 17.6 chunks) the overlap wins, 55.37 against 55.42 ms.
 int lane_count(const srk_workspace_s* w, int np1, int np2, int S2, int ntiles)
 {
     size_t one = lane_capacity(w, np1, np2, S2, 1);
     if (w->chunk_override > 0) one = std::min<size_t>(one, (size_t)w->chunk_override);
-    return w->concurrent && (size_t)ntiles > 7 * one && lane_capacity(w, np1, np2, S2, 2) >= 1
+    return w->concurrent && (size_t)ntiles > 15 * one && lane_capacity(w, np1, np2, S2, 2) >= 1
                ? 2
                : 1;
 }

@@ -71,7 +71,7 @@ def test_two_stream_adjoint_equals_one_stream():
     g = dev(patterns.grf(N, 22, B=B))
     v = dev(np.random.default_rng(3).standard_normal((B, W, W)))
     ws = srk.Workspace(N, W, e, M, B_max=B)
-    ws.set_tuning(chunk_tiles=3)  # 27 tiles per call: nine chunks, so the calls take both lanes
+    ws.set_tuning(chunk_tiles=1)  # 27 tiles per call in one-tile chunks: the calls take both lanes
     out, hs = {}, {}
     for conc in (0, 1):
         ws.set_tuning(concurrency=conc)
