@@ -15,7 +15,10 @@ pytestmark = pytest.mark.gpu
 def test_candidate_tiles_respect_the_halo():
     assert autotune.candidate_tiles(512, 121) == [256, 512]      # 256 > 2*60 + 2
     assert autotune.candidate_tiles(200, 31) == [256]            # whole image only
-    assert autotune.candidate_tiles(4096, 121) == [256, 512, 1024, 2048]
+    # 2048-point overlap-and-save tiles only where 1024-point ones cannot hold the halo
+    assert autotune.candidate_tiles(4096, 121) == [256, 512, 1024]
+    assert autotune.candidate_tiles(4096, 1023) == [2048]          # 1024 <= 2*511 + 2
+    assert autotune.candidate_tiles(2000, 121) == [256, 512, 1024, 2048]  # 2048: whole image
 
 
 def test_autotune_picks_a_working_configuration(orc, tmp_path):
